@@ -1,0 +1,174 @@
+/*
+ * vcgpu.h -- C ABI of the MI355X (gfx950) branch-and-reduce vertex-cover solver.
+ *
+ * The reference has no FFI layer: its boundary is main()'s launch block
+ * (src/main.cu:300-344) handing by-value structs of device pointers to eight
+ * __global__ kernels, plus the host helpers it calls before and after.  Each
+ * entry point below replaces one of those pieces; the comment on each cites
+ * the reference code it stands in for.  Plain C types only, no exit(): every
+ * call returns a vc_status.
+ *
+ * Threading: one vc_ctx per host thread; calls on one ctx are serialised by
+ * the caller.  A vc_solve call blocks until its persistent kernel finishes.
+ */
+#ifndef VCGPU_H
+#define VCGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    VC_OK = 0,
+    VC_EINVAL = 1,     /* bad argument (reference: exit(0) in src/config.h:104-105, 143-144) */
+    VC_ENOMEM = 2,     /* device memory does not fit (reference: src/auxFunctions.cpp:399-402) */
+    VC_EDEVICE = 3,    /* HIP error (reference: "GPU Error" exit(1), src/main.cu:348-352) */
+    VC_ETIMEOUT = 4,   /* node budget or spin bound exhausted before the search finished */
+    VC_ECANCELLED = 5, /* cancelled by the caller */
+    VC_ENOTSUP = 6,    /* combination not supported */
+    VC_EBADGRAPH = 7   /* duplicate edge / self-loop / index out of range
+                          (reference: assert(check_graph), src/auxFunctions.cpp:358) */
+} vc_status;
+
+/* src/config.h:125-129 */
+typedef enum { VC_MVC = 0, VC_PVC = 1 } vc_instance;
+
+/* src/config.h:81-86 (SEQUENTIAL is the CPU path and has no device entry). */
+typedef enum { VC_STACK_ONLY = 0, VC_HYBRID = 1 } vc_version;
+
+/* CSR graph as src/CSRGraphRep.h:4-18 lays it out: srcPtr[vertexNum+1],
+ * dst[2*edgeNum] sorted ascending per vertex, degree[v] = srcPtr[v+1]-srcPtr[v].
+ * (srcPtrUncompressed is used only by the out-of-scope matching kernel.) */
+typedef struct {
+    uint32_t vertexNum;
+    uint32_t edgeNum;
+    const uint32_t *srcPtr;
+    const uint32_t *dst;
+    const int32_t *degree;
+} vc_csr;
+
+/* Per-graph job: what one search is asked.  upper_bound plays the role of
+ * `minimum` in src/main.cu:73 (a valid cover size U from the host
+ * approximations); for MVC the search looks for covers smaller than U and
+ * returns U when none exists, as src/Sequential.cpp:67-77 does. */
+typedef struct {
+    uint32_t instance;    /* vc_instance */
+    uint32_t k;           /* PVC parameter (-k, src/config.h:211-213) */
+    uint32_t upper_bound; /* U; must be the size of some vertex cover */
+    uint32_t reserved;
+    uint64_t max_nodes;   /* per-graph tree-node budget, 0 = unlimited */
+} vc_job;
+
+/* Launch-level parameters: the reference's Config (src/config.h:162-176)
+ * fields that shape the device solve.  0 / -1 mean "pick automatically". */
+typedef struct {
+    uint32_t version;           /* vc_version (-v) */
+    uint32_t block_dim;         /* threads per workgroup (-b); 0 = auto */
+    uint32_t num_blocks;        /* workgroups (-n); 0 = auto (occupancy x CUs) */
+    int32_t use_global_memory;  /* -g: -1 auto, 0 degree arrays in LDS, 1 in HBM */
+    uint32_t list_size;         /* global worklist slots (-q, power of 2); 0 = auto */
+    float list_threshold;       /* donate while queued < threshold*size (-t); <=0 = auto */
+    uint32_t start_depth;       /* static split depth for VC_STACK_ONLY (-d) */
+    uint32_t time_limit_ms;     /* wall-clock limit of the launch; 0 = one hour.  When it
+                                   passes every worker drains and exits, and unfinished
+                                   graphs report VC_ETIMEOUT. */
+    uint32_t flags;             /* reserved, 0 */
+} vc_params;
+
+/* Per-graph outcome.  Replaces the 4-byte minimum_d / kFound_d readback of
+ * src/main.cu:355-359 and adds the node count and status. */
+typedef struct {
+    uint32_t minimum;        /* MVC: cover size found (== MVC when solved);
+                                PVC: size of the cover found when k_found */
+    uint32_t k_found;        /* PVC: 1 = a cover of size <= k exists */
+    uint32_t solved;         /* 1 = search finished (answer exact) */
+    uint32_t cover_from_gpu; /* 1 = cover[] was written by the device search;
+                                0 = no cover < U exists (MVC) and cover[] is
+                                untouched (the caller holds the U-cover) */
+    uint64_t tree_nodes;     /* node-loop iterations spent on this graph */
+    int32_t status;          /* vc_status for this graph */
+    int32_t reserved;
+} vc_result;
+
+/* Launch facts of the last vc_solve (the reference prints these,
+ * src/main.cu:200-204, and writes them to Results.csv, src/auxFunctions.cpp:589-593). */
+typedef struct {
+    uint32_t num_blocks;
+    uint32_t block_dim;
+    uint32_t workers;
+    uint32_t blocks_per_cu;
+    uint32_t use_global_memory;
+    uint32_t list_size;
+    uint32_t list_threshold;
+    uint32_t stack_depth;
+    float kernel_ms;
+    uint32_t cu_count;
+    uint64_t tree_nodes;
+} vc_launch_info;
+
+typedef struct vc_ctx vc_ctx;
+typedef struct vc_graphs vc_graphs;
+
+/* Device context on one GPU.  Replaces the device queries of
+ * src/main.cu:101-121. */
+vc_status vc_ctx_create(int device, vc_ctx **out);
+vc_status vc_ctx_destroy(vc_ctx *ctx);
+
+/* Upload `count` graphs into one device-resident graph set (one HBM buffer).
+ * count == 1 replaces allocateGraph (src/CSRGraphRep.cuh:3-27); larger
+ * counts are the batched form.  Graphs are checked (check_graph,
+ * src/auxFunctions.cpp:308-354) and rejected with VC_EBADGRAPH. */
+vc_status vc_graphs_upload(vc_ctx *ctx, int count, const vc_csr *graphs, vc_graphs **out);
+vc_status vc_graphs_free(vc_ctx *ctx, vc_graphs *set);
+
+/* Solve every graph of the set in ONE persistent launch: jobs[count],
+ * results[count]; covers == NULL or covers[i] == NULL skips that cover,
+ * otherwise covers[i] receives vertexNum bytes (1 = in cover).
+ * Replaces src/main.cu:219-366 (stacks, worklist, launch, readback) for the
+ * four kernels GlobalWorkList{,Parameterized}_{shared,global}_kernel
+ * (src/GlobalWorkList.cuh:12-18, src/GlobalWorkListParameterized.cuh:12-18) and
+ * LocalStacks{,Parameterized}_{shared,global}_kernel
+ * (src/LocalStacks.cuh:10-16, src/LocalStacksParameterized.cuh:10-16). */
+vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *params,
+                   vc_result *results, uint8_t *const *covers, vc_launch_info *info);
+
+/* Workgroups per CU the solve would run with (replaces
+ * cudaOccupancyMaxActiveBlocksPerMultiprocessor, src/main.cu:131-191). */
+vc_status vc_occupancy(vc_ctx *ctx, const vc_graphs *set, const vc_params *params, int *blocks_per_cu,
+                       int *block_dim, int *use_global_memory);
+
+/* ---- host helpers the reference's driver calls around the solve ---- */
+
+/* Owned CSR filled by vc_load_graph_file; free with vc_csr_free. */
+typedef struct {
+    vc_csr csr;
+    void *storage;
+} vc_csr_owned;
+
+/* createCSRGraphFromFile (src/auxFunctions.cpp:129-209): header "n m" then
+ * m 1-indexed edges; adjacency sorted.  Errors are returned, not asserted. */
+vc_status vc_load_graph_file(const char *path, vc_csr_owned *out);
+/* Same CSR from a 0-indexed edge list. */
+vc_status vc_csr_from_edges(uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *ev, vc_csr_owned *out);
+void vc_csr_free(vc_csr_owned *g);
+
+/* check_graph (src/auxFunctions.cpp:308-354): 1 = valid. */
+int vc_check_graph(const vc_csr *g);
+
+/* RemoveMaxApproximateMVC (src/auxFunctions.cpp:211-256).  Returns U; if
+ * cover != NULL it receives the greedy cover (vertexNum bytes). */
+uint32_t vc_approx_remove_max(const vc_csr *g, uint8_t *cover);
+
+/* RemoveEdgeApproximateMVC (src/auxFunctions.cpp:265-306); the reference
+ * seeds with time(0) inside each random draw, here the seed is explicit. */
+uint32_t vc_approx_remove_edge(const vc_csr *g, uint32_t seed, uint8_t *cover);
+
+const char *vc_status_string(vc_status s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VCGPU_H */

@@ -54,12 +54,24 @@ def build() -> None:
 _lib = None
 
 
+def load_library(path: str):
+    """ctypes handle of an oracle build at `path` with every signature set."""
+    L = C.CDLL(path)
+    _configure(L)
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
+        _lib = load_library(_LIB_PATH)
+    return _lib
+
+
+def _configure(L):
+    if True:
         L.or_graph_load.argtypes = [C.c_char_p, C.POINTER(_OrGraph)]
         L.or_graph_from_edges.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p, C.POINTER(_OrGraph)]
         L.or_graph_free.argtypes = [C.POINTER(_OrGraph)]
@@ -76,8 +88,6 @@ def lib():
         L.or_cover_is_valid.argtypes = [C.POINTER(_OrGraph), _u8p]
         L.or_solve_batch_mt.argtypes = [C.c_int, C.POINTER(_OrGraph), C.c_int, C.c_uint64, _u32p, _u64p, _i32p]
         L.or_solve_batch_mt.restype = C.c_double
-        _lib = L
-    return _lib
 
 
 @dataclass

@@ -16,6 +16,13 @@
 #include <string.h>
 #include <time.h>
 
+/* Mutation switch for the oracle's own self-test (tests/test_cpu_oracle.py
+ * builds variants with -DORACLE_PLANT=N and checks that the golden vectors
+ * reject each one).  The shipped library is built with 0. */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+
 /* ------------------------------------------------------------------ */
 /* Graph construction                                                  */
 /* ------------------------------------------------------------------ */
@@ -259,7 +266,7 @@ uint32_t or_remove_max_approx(const or_graph *g)
         uint32_t maxV = 0;
         int32_t maxD = 0;
         for (uint32_t i = 0; i < a.n; i++) {
-            if (a.degree[i] > maxD) {
+            if (ORACLE_PLANT == 5 ? (a.degree[i] >= maxD && a.degree[i] > 0) : a.degree[i] > maxD) {
                 maxV = i;
                 maxD = a.degree[i];
             }
@@ -359,7 +366,8 @@ static int rule_high(const or_graph *g, int32_t *deg, uint32_t *nd, int minimum)
     do {
         changed = 0;
         for (uint32_t i = 0; i < g->n; ++i) {
-            if (deg[i] > 0 && (uint32_t)deg[i] + *nd >= (uint32_t)minimum) {
+            if (deg[i] > 0 && (ORACLE_PLANT == 4 ? (uint32_t)deg[i] + *nd > (uint32_t)minimum
+                                                  : (uint32_t)deg[i] + *nd >= (uint32_t)minimum)) {
                 changed = 1;
                 deleted = 1;
                 del_vertex(g, i, deg, nd);
@@ -401,7 +409,7 @@ static int rule_triangle(const or_graph *g, int32_t *deg, uint32_t *nd)
                     changed = 1;
                     deleted = 1;
                     del_vertex(g, n1, deg, nd);
-                    del_vertex(g, n2, deg, nd);
+                    if (ORACLE_PLANT != 3) del_vertex(g, n2, deg, nd);
                     break;
                 }
             }
@@ -471,7 +479,7 @@ static uint32_t seq_search(const or_graph *g, uint32_t minimum, int param, uint3
         uint32_t numEdges = 0;
         for (uint32_t i = 0; i < n; ++i) {
             int32_t d = deg[i];
-            if (d > maxDegree) {
+            if (ORACLE_PLANT == 2 ? (d >= maxDegree && d > 0) : d > maxDegree) {
                 maxDegree = d;
                 maxVertex = i;
             }
@@ -485,7 +493,7 @@ static uint32_t seq_search(const or_graph *g, uint32_t minimum, int param, uint3
             prune = nd >= minimum;
             if (!prune) {
                 st->square_calls++;
-                prune = (long long)numEdges >= sq((int)(minimum - nd - 1)) + 1;
+                prune = (long long)numEdges >= sq((int)(minimum - nd - (ORACLE_PLANT == 1 ? 0u : 1u))) + 1;
             }
         } else {
             /* src/SequentialParameterized.cpp:70 */

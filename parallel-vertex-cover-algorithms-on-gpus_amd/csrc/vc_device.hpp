@@ -1,0 +1,103 @@
+// vc_device.hpp -- data layout shared by the gfx950 kernels and their host
+// launcher.  All structures live in HBM and are written by the host launcher
+// (init kernel) and the persistent solve kernel only.
+#pragma once
+
+#include <stdint.h>
+
+namespace vcg {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kWave = 64;
+
+// One graph of a device graph set.  Offsets are in 32-bit words into the set
+// buffer (srcPtr[n+1], dst[2m], degree[n]).  Same arrays as the reference's
+// CSRGraph (src/CSRGraphRep.h:4-18).
+struct GDesc {
+    uint32_t n, m;
+    uint32_t off_src, off_dst, off_deg;
+    uint32_t off_cover;  // first 32-bit word of this graph's cover bitmap
+    uint32_t instance;   // 0 MVC, 1 PVC
+    uint32_t k;
+    uint32_t U;          // upper bound (valid cover size)
+    uint32_t pad0;
+    uint64_t max_nodes;  // 0 = unlimited
+    uint32_t pad1[4];
+};
+static_assert(sizeof(GDesc) == 64, "GDesc is one 64-byte line");
+
+// Mutable search state of one graph (one 64-byte line, so graphs do not
+// share lines).
+struct GState {
+    int32_t best;         // MVC incumbent; starts at U (src/main.cu:73 with VC 1)
+    uint32_t found;       // PVC: a cover <= k was found (kFound_d, src/main.cu:249-251)
+    uint32_t stop;        // 1 = drop remaining work (found or budget exhausted)
+    uint32_t outstanding; // live subtrees: queued items + owned by a worker
+    uint64_t nodes;       // tree nodes spent on this graph
+    uint32_t cover_lock;
+    int32_t cover_size;   // size of the cover held in the bitmap, -1 = none
+    uint32_t done;        // outstanding reached 0
+    uint32_t budget_hit;  // stop was set by the node budget
+    uint32_t pad[6];
+};
+static_assert(sizeof(GState) == 64, "GState is one 64-byte line");
+
+// Launch-wide counters (each on its own line).
+struct Global {
+    // Packed termination word [enqueued:32 | registered:16 | waiting:16];
+    // updated with single 64-bit atomics so a dequeue moves a worker out of
+    // "waiting" and an item out of "enqueued" in one step (the reference packs
+    // {numWaiting, numEnqueued} the same way, src/BWDWorkList.cuh:8-11,200-205).
+    unsigned long long counter;
+    uint32_t pad0[14];
+    unsigned long long head;  // 64-bit ring positions (the reference uses two
+    uint32_t pad1[14];        // 32-bit halves of one word, src/BWDWorkList.cuh:63-69)
+    unsigned long long tail;
+    uint32_t pad2[14];
+    int32_t count;            // occupancy guard (src/BWDWorkList.cuh:78-110)
+    uint32_t pad3[15];
+    uint32_t next_root;       // root / static-path hand-out counter
+    uint32_t pad4[15];
+    uint32_t done_flag;       // set once termination is observed
+    uint32_t error;           // nonzero: an internal bound was hit (see kErr*)
+    uint32_t cancel;          // cancel request (host, or the deadline)
+    uint32_t timed_out;       // the deadline passed before the search finished
+    uint64_t t0;              // wall clock at init (device clock ticks)
+    uint32_t pad5[10];
+};
+
+constexpr unsigned long long kWaitOne = 1ull;
+constexpr unsigned long long kRegOne = 1ull << 16;
+constexpr unsigned long long kEnqOne = 1ull << 32;
+
+// glob->error codes
+constexpr uint32_t kErrStackOverflow = 5;
+
+struct SolveArgs {
+    const uint32_t *gbuf;   // graph set words
+    const GDesc *gdesc;
+    GState *gstate;
+    uint32_t *covers;       // cover bitmaps
+    Global *glob;
+    int32_t *stk;           // [workers][depth][npad]
+    uint2 *stk_meta;        // [workers][depth] = (d, branch vertex)
+    int32_t *q_deg;         // [qsize][npad]
+    uint4 *q_meta;          // [qsize] = (graph, d, branch vertex, 0)
+    uint32_t *tickets;      // [qsize]
+    uint64_t *worker_nodes; // [workers]
+    uint32_t num_graphs;
+    uint32_t npad;          // slot stride in ints (multiple of 16)
+    uint32_t mwords;        // mark-bitset words per worker (multiple of 4)
+    uint32_t depth;         // stack slots per worker
+    uint32_t qsize;         // power of two
+    uint32_t qthreshold;    // donate while enqueued < qthreshold
+    uint32_t workers;
+    uint32_t version;       // 0 static split (stack only), 1 hybrid
+    uint32_t start_depth;   // static split depth
+    uint32_t roots;         // number of root items handed out by next_root
+    uint64_t limit_ticks;   // deadline = glob->t0 + limit_ticks (device wall clock)
+    uint32_t hbm_mode;      // 1: degree arrays in HBM scratch instead of LDS
+    int32_t *hbm_scratch;   // [workers][2*npad + mwords] when hbm_mode
+};
+
+}  // namespace vcg

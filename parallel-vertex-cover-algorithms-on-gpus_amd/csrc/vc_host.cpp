@@ -1,0 +1,284 @@
+// vc_host.cpp -- host-side helpers of the reference driver, behind the C ABI:
+// graph file loader, graph check and the two upper-bound approximations.
+// They run before the device solve (src/main.cu:49-73).
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "vcgpu.h"
+
+namespace {
+
+struct OwnedCsr {
+    std::vector<uint32_t> src, dst;
+    std::vector<int32_t> deg;
+};
+
+// CSR build of createCSRGraphFromFile (src/auxFunctions.cpp:158-206):
+// degree count, prefix offsets, each edge stored in both directions, then
+// every adjacency sorted ascending.
+vc_status build_csr(uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *ev, vc_csr_owned *out)
+{
+    for (uint32_t i = 0; i < m; ++i)
+        if (eu[i] >= n || ev[i] >= n) return VC_EBADGRAPH;
+    OwnedCsr *o = new OwnedCsr();
+    o->src.assign((size_t)n + 1, 0u);
+    o->dst.assign(2ull * m + 1, 0u);
+    o->deg.assign((size_t)n + 1, 0);
+    for (uint32_t i = 0; i < m; ++i) {
+        o->deg[eu[i]]++;
+        o->deg[ev[i]]++;
+    }
+    std::vector<uint32_t> fill((size_t)n + 1);
+    uint32_t next = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        o->src[v] = next;
+        fill[v] = next;
+        next += (uint32_t)o->deg[v];
+    }
+    o->src[n] = 2u * m;
+    for (uint32_t i = 0; i < m; ++i) {
+        o->dst[fill[eu[i]]++] = ev[i];
+        o->dst[fill[ev[i]]++] = eu[i];
+    }
+    for (uint32_t v = 0; v < n; ++v) std::sort(o->dst.begin() + o->src[v], o->dst.begin() + o->src[v + 1]);
+    out->storage = o;
+    out->csr.vertexNum = n;
+    out->csr.edgeNum = m;
+    out->csr.srcPtr = o->src.data();
+    out->csr.dst = o->dst.data();
+    out->csr.degree = o->deg.data();
+    return VC_OK;
+}
+
+// Working copy for the approximations.  The reference edits the graph in
+// place (CSRGraph::deleteVertex, src/CSRGraphRep.cpp:46-64): a deleted
+// vertex is removed from the live prefix of each neighbour's list by
+// overwriting it with that list's last live entry.  The approximations read
+// the lists after such edits (full original ranges, unsorted prefixes), so
+// the same edit is kept here to obtain the same U.
+struct Approx {
+    uint32_t n;
+    std::vector<uint32_t> src, dst;
+    std::vector<int32_t> deg;
+
+    explicit Approx(const vc_csr &g) : n(g.vertexNum)
+    {
+        src.assign(g.srcPtr, g.srcPtr + n + 1);
+        dst.assign(g.dst, g.dst + 2ull * g.edgeNum);
+        deg.assign(g.degree, g.degree + n);
+    }
+
+    void remove(uint32_t v)
+    {
+        const int32_t vb = (int32_t)src[v], ve = (int32_t)src[v] + deg[v];
+        for (int32_t i = vb; i < ve; ++i) {
+            const uint32_t nb = dst[i];
+            const int32_t last = (int32_t)src[nb] + deg[nb];
+            for (int32_t j = (int32_t)src[nb]; j < last; ++j) {
+                if (dst[j] == v) {
+                    dst[j] = dst[last - 1];
+                    if (deg[nb] != -1) deg[nb]--;
+                }
+            }
+        }
+        deg[v] = -1;
+    }
+
+    // Membership search over a range that may be unsorted after edits
+    // (auxBinarySearch, src/auxFunctions.cpp:25-42).  The reference's
+    // unsigned r = mid - 1 would wrap at mid == 0 and read out of bounds;
+    // here the search stops there instead.
+    bool search(uint32_t l, uint32_t r, uint32_t x) const
+    {
+        while (l <= r) {
+            const uint32_t mid = l + (r - l) / 2;
+            if (dst[mid] == x) return true;
+            if (dst[mid] < x) {
+                l = mid + 1;
+            } else {
+                if (mid == 0) return false;
+                r = mid - 1;
+            }
+        }
+        return false;
+    }
+
+    // leafReductionRule(CSRGraph&, unsigned&) (src/auxFunctions.cpp:44-69)
+    void leaf(uint32_t &count)
+    {
+        bool changed;
+        do {
+            changed = false;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (deg[i] != 1) continue;
+                changed = true;
+                for (uint32_t j = src[i]; j < src[i + 1]; ++j) {
+                    if (deg[dst[j]] != -1) {
+                        remove(dst[j]);
+                        ++count;
+                    }
+                }
+            }
+        } while (changed);
+    }
+
+    // triangleReductionRule(CSRGraph&, unsigned&) (src/auxFunctions.cpp:71-127)
+    void triangle(uint32_t &count)
+    {
+        bool changed;
+        do {
+            changed = false;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (deg[i] != 2) continue;
+                uint32_t a = 0, b = 0;
+                bool have_a = false, keep = false;
+                for (uint32_t e = src[i]; e < src[i + 1]; ++e) {
+                    const uint32_t u = dst[e];
+                    const int32_t du = deg[u];
+                    if (du <= 0) continue;
+                    if (du == 1 || (du == 2 && u < i)) {
+                        keep = true;
+                        break;
+                    }
+                    if (!have_a) {
+                        have_a = true;
+                        a = u;
+                    } else {
+                        b = u;
+                        break;
+                    }
+                }
+                if (!keep && search(src[b], src[b + 1] - 1, a)) {
+                    remove(a);
+                    remove(b);
+                    count += 2;
+                    changed = true;
+                    break;
+                }
+            }
+        } while (changed);
+    }
+
+    void cover(uint8_t *out) const
+    {
+        if (!out) return;
+        for (uint32_t v = 0; v < n; ++v) out[v] = deg[v] == -1;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+vc_status vc_csr_from_edges(uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *ev, vc_csr_owned *out)
+{
+    if (!out || (m && (!eu || !ev))) return VC_EINVAL;
+    std::memset(out, 0, sizeof(*out));
+    return build_csr(n, m, eu, ev, out);
+}
+
+vc_status vc_load_graph_file(const char *path, vc_csr_owned *out)
+{
+    if (!path || !out) return VC_EINVAL;
+    std::memset(out, 0, sizeof(*out));
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return VC_EINVAL;
+    unsigned n = 0, m = 0;
+    if (std::fscanf(fp, "%u%u", &n, &m) != 2) {
+        std::fclose(fp);
+        return VC_EBADGRAPH;
+    }
+    std::vector<uint32_t> eu(m), ev(m);
+    for (unsigned i = 0; i < m; ++i) {
+        unsigned a, b;
+        if (std::fscanf(fp, "%u%u", &a, &b) != 2 || a == 0 || b == 0) {
+            std::fclose(fp);
+            return VC_EBADGRAPH;
+        }
+        eu[i] = a - 1u;  // 1-indexed file (src/auxFunctions.cpp:151-153)
+        ev[i] = b - 1u;
+    }
+    std::fclose(fp);
+    return build_csr(n, m, eu.data(), ev.data(), out);
+}
+
+void vc_csr_free(vc_csr_owned *g)
+{
+    if (!g) return;
+    delete static_cast<OwnedCsr *>(g->storage);
+    std::memset(g, 0, sizeof(*g));
+}
+
+int vc_check_graph(const vc_csr *g)
+{
+    if (!g) return 0;
+    for (uint32_t v = 0; v < g->vertexNum; ++v) {
+        for (uint32_t e = g->srcPtr[v]; e < g->srcPtr[v + 1]; ++e) {
+            if (g->dst[e] == v || g->dst[e] >= g->vertexNum) return 0;
+            if (e > g->srcPtr[v] && g->dst[e] == g->dst[e - 1]) return 0;
+        }
+    }
+    return 1;
+}
+
+// RemoveMaxApproximateMVC (src/auxFunctions.cpp:211-256).  The reference's
+// rule functions return the exit value of their own loop (always false), so
+// leaf and triangle run once per outer step before the max-degree deletion.
+uint32_t vc_approx_remove_max(const vc_csr *g, uint8_t *cover)
+{
+    if (!g) return 0;
+    Approx a(*g);
+    uint32_t count = 0;
+    for (;;) {
+        a.leaf(count);
+        a.triangle(count);
+        uint32_t best_v = 0;
+        int32_t best_d = 0;
+        for (uint32_t v = 0; v < a.n; ++v) {
+            if (a.deg[v] > best_d) {
+                best_d = a.deg[v];
+                best_v = v;
+            }
+        }
+        if (best_d == 0) break;
+        a.remove(best_v);
+        ++count;
+    }
+    a.cover(cover);
+    return count;
+}
+
+// RemoveEdgeApproximateMVC (src/auxFunctions.cpp:265-306): walk the vertices
+// in order, reduce, then put both ends of one random live edge of the vertex
+// into the cover.  getRandom (src/auxFunctions.cpp:258-263) re-seeds with
+// time(0) on every draw; the seed is explicit here.
+uint32_t vc_approx_remove_edge(const vc_csr *g, uint32_t seed, uint8_t *cover)
+{
+    if (!g) return 0;
+    Approx a(*g);
+    uint32_t count = 0;
+    uint32_t remaining = g->edgeNum;
+    for (uint32_t v = 0; v < a.n && remaining > 0; ++v) {
+        a.leaf(count);
+        a.triangle(count);
+        if (a.deg[v] <= 0) continue;
+        const int lower = (int)a.src[v];
+        const int upper = (int)(a.src[v] + (uint32_t)a.deg[v] - 1u);
+        std::srand(seed);
+        const uint32_t pick = (uint32_t)((std::rand() % (upper - lower + 1)) + lower);
+        const uint32_t u = a.dst[pick];
+        remaining -= (uint32_t)a.deg[v];
+        remaining -= (uint32_t)a.deg[u];
+        ++remaining;
+        a.remove(v);
+        a.remove(u);
+        count += 2;
+    }
+    a.cover(cover);
+    return count;
+}
+
+}  // extern "C"

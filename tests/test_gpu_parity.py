@@ -1,0 +1,149 @@
+"""GPU parity: the HIP solver (libvcgpu.so through its C ABI) against the CPU
+oracle (a restatement of the reference's src/Sequential.cpp and
+src/SequentialParameterized.cpp) and against independent exact answers.
+
+Bar: the minimum cover size is bit-exact, the PVC yes/no matches [k >= MVC],
+and every returned cover is a valid cover of the returned size.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import graphgen
+import oracle as O
+import vcgpu
+
+pytestmark = pytest.mark.gpu
+
+
+def P(**kw):
+    """Launch parameters with a 60 s device-side time limit."""
+    kw.setdefault("time_limit_ms", 60000)
+    return vcgpu.Params(**kw)
+
+
+def brute_mvc(n, edges):
+    """Exact MVC by enumerating vertex subsets (n <= 18)."""
+    edges = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    if len(edges) == 0:
+        return 0
+    masks = np.arange(1 << n, dtype=np.int64)
+    ok = np.ones(1 << n, dtype=bool)
+    for u, v in edges:
+        ok &= (((masks >> u) & 1) | ((masks >> v) & 1)).astype(bool)
+    pop = np.array([bin(x).count("1") for x in range(1 << n)])
+    return int(pop[ok].min())
+
+
+def special_graphs():
+    out = []
+    out.append(("empty3", 3, []))
+    out.append(("single_vertex", 1, []))
+    out.append(("one_edge", 2, [(0, 1)]))
+    out.append(("two_edges_disjoint", 4, [(0, 1), (2, 3)]))
+    out.append(("path7", 7, [(i, i + 1) for i in range(6)]))
+    out.append(("cycle9", 9, [(i, (i + 1) % 9) for i in range(9)]))
+    out.append(("star12", 13, [(0, i) for i in range(1, 13)]))
+    out.append(("k6", 6, list(itertools.combinations(range(6), 2))))
+    out.append(("k3_4", 7, [(a, b) for a in range(3) for b in range(3, 7)]))
+    petersen = [(i, (i + 1) % 5) for i in range(5)] + [(i, i + 5) for i in range(5)] + \
+               [(5 + i, 5 + (i + 2) % 5) for i in range(5)]
+    out.append(("petersen", 10, petersen))
+    out.append(("triangle_pendant", 5, [(0, 1), (1, 2), (0, 2), (2, 3), (3, 4)]))
+    return out
+
+
+def random_tiny(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(2, 15))
+        p = float(rng.uniform(0.1, 0.7))
+        e = graphgen.erdos_renyi(n, p, seed * 1000 + i)
+        out.append((f"er{n}_{i}", n, e))
+    return out
+
+
+def _closed_forms():
+    return {"empty3": 0, "single_vertex": 0, "one_edge": 1, "two_edges_disjoint": 2, "path7": 3, "cycle9": 5,
+            "star12": 1, "k6": 5, "k3_4": 3, "petersen": 6, "triangle_pendant": 3}
+
+
+def test_special_graphs_closed_forms(ctx):
+    cf = _closed_forms()
+    for name, n, e in special_graphs():
+        g = vcgpu.csr_from_edges(n, e)
+        mvc, cover, r, info = vcgpu.solve_mvc(ctx, g, P())
+        assert r.solved, name
+        assert mvc == cf[name], name
+        assert O.cover_is_valid(O.from_edges(n, e), cover), name
+        assert int(np.sum(cover)) == mvc, name
+
+
+def test_tiny_batch_matches_bruteforce(ctx):
+    cases = special_graphs() + random_tiny(80, 7)
+    graphs = [vcgpu.csr_from_edges(n, e) for _, n, e in cases]
+    ubs = [vcgpu.upper_bound(g, seed=3) for g in graphs]
+    gs = ctx.upload(graphs)
+    res, info, st = ctx.solve(gs, [vcgpu.Job(vcgpu.VC_MVC, 0, u, 0) for u, _ in ubs], P())
+    assert st == vcgpu.VC_OK
+    for (name, n, e), (u, ucov), r in zip(cases, ubs, res):
+        exact = brute_mvc(n, e)
+        assert r.solved, name
+        assert r.minimum == exact, (name, r.minimum, exact)
+        cover = r.cover if r.cover is not None else ucov
+        assert O.cover_is_valid(O.from_edges(n, e), cover), name
+        assert int(np.sum(cover)) == exact, name
+
+
+@pytest.mark.parametrize("version", [vcgpu.VC_HYBRID, vcgpu.VC_STACK_ONLY])
+def test_tiny_pvc_answers(ctx, version):
+    cases = random_tiny(40, 11)
+    graphs, jobs, expect = [], [], []
+    for name, n, e in cases:
+        g = vcgpu.csr_from_edges(n, e)
+        exact = brute_mvc(n, e)
+        U, _ = vcgpu.upper_bound(g, seed=5)
+        for k in (exact - 1, exact, exact + 1):
+            if k < 0:
+                continue
+            graphs.append(g)
+            jobs.append(vcgpu.Job(vcgpu.VC_PVC, k, U, 0))
+            expect.append((name, n, e, k, int(k >= exact)))
+    gs = ctx.upload(graphs)
+    res, info, st = ctx.solve(gs, jobs, P(version=version, start_depth=2))
+    assert st == vcgpu.VC_OK
+    for (name, n, e, k, want), r in zip(expect, res):
+        assert r.solved
+        assert r.k_found == want, (name, k, r.k_found, want)
+        if r.k_found:
+            assert r.cover is not None
+            assert O.cover_is_valid(O.from_edges(n, e), r.cover)
+            assert int(np.sum(r.cover)) <= k
+
+
+def _medium_cases():
+    out = []
+    for s in range(6):
+        out.append((f"er90_{s}", 90, graphgen.erdos_renyi(90, 0.06, 100 + s)))
+    for s in range(3):
+        out.append((f"er120_{s}", 120, graphgen.erdos_renyi(120, 0.05, 200 + s)))
+    for s in range(3):
+        out.append((f"ba300_{s}", 300, graphgen.barabasi_albert(300, 3, 300 + s)))
+    for s in range(4):
+        out.append((f"reg3_{s}", 120, graphgen.regular3_rewired(120, s)))
+    return out
+
+
+@pytest.mark.parametrize("version", [vcgpu.VC_HYBRID, vcgpu.VC_STACK_ONLY])
+def test_medium_graphs_match_oracle(ctx, version):
+    for name, n, e in _medium_cases():
+        og = O.from_edges(n, e)
+        want, _, _ = O.mvc(og)
+        g = vcgpu.csr_from_edges(n, e)
+        mvc, cover, r, info = vcgpu.solve_mvc(ctx, g, P(version=version, start_depth=4))
+        assert r.solved, name
+        assert mvc == want, (name, mvc, want)
+        assert O.cover_is_valid(og, cover), name
+        assert int(np.sum(cover)) == want, name
