@@ -73,21 +73,26 @@ constexpr unsigned long long kEnqOne = 1ull << 32;
 // glob->error codes
 constexpr uint32_t kErrStackOverflow = 5;
 
+// Degree arrays are stored as DegT (int16 when every graph of the launch
+// has fewer than 32768 vertices, else int32); a node slot on a stack or in
+// the worklist is npad DegT values, a multiple of 64 bytes.
 struct SolveArgs {
     const uint32_t *gbuf;   // graph set words
     const GDesc *gdesc;
     GState *gstate;
     uint32_t *covers;       // cover bitmaps
     Global *glob;
-    int32_t *stk;           // [workers][depth][npad]
+    void *stk;              // [workers][depth][npad] DegT
     uint2 *stk_meta;        // [workers][depth] = (d, branch vertex)
-    int32_t *q_deg;         // [qsize][npad]
+    void *q_deg;            // [qsize][npad] DegT
     uint4 *q_meta;          // [qsize] = (graph, d, branch vertex, 0)
     uint32_t *tickets;      // [qsize]
     uint64_t *worker_nodes; // [workers]
     uint32_t num_graphs;
-    uint32_t npad;          // slot stride in ints (multiple of 16)
+    uint32_t npad;          // slot length in DegT values
+    uint32_t lcap;          // mark-list capacity per worker (multiple of 4)
     uint32_t mwords;        // mark-bitset words per worker (multiple of 4)
+    uint32_t worker_bytes;  // bytes of per-worker node storage (LDS or HBM)
     uint32_t depth;         // stack slots per worker
     uint32_t qsize;         // power of two
     uint32_t qthreshold;    // donate while enqueued < qthreshold
@@ -96,8 +101,7 @@ struct SolveArgs {
     uint32_t start_depth;   // static split depth
     uint32_t roots;         // number of root items handed out by next_root
     uint64_t limit_ticks;   // deadline = glob->t0 + limit_ticks (device wall clock)
-    uint32_t hbm_mode;      // 1: degree arrays in HBM scratch instead of LDS
-    int32_t *hbm_scratch;   // [workers][2*npad + mwords] when hbm_mode
+    char *hbm_nodes;        // [workers][worker_bytes] node storage in HBM mode
 };
 
 }  // namespace vcg

@@ -3,8 +3,11 @@
 //
 // One wave64 is one search worker.  A workgroup holds several independent
 // workers; no workgroup barrier is ever used.  Each worker keeps the degree
-// array of its current tree node in LDS (int32, -1 = vertex in the cover),
-// plus a mark bitset and a compact mark list used by the reduction rules.
+// array of its current tree node (-1 = vertex in the cover) in LDS -- or, for
+// graphs too large for LDS, in a per-worker HBM scratch area (the reference's
+// `_global_` kernels) -- plus a mark bitset and a compact mark list used by
+// the reduction rules.  Degrees are int16 when every graph of the launch has
+// fewer than 32768 vertices, halving LDS use and stack/worklist traffic.
 //
 // Reference semantics kept (file:line in the reference):
 //   leaf rule            src/helperFunctions.cuh:229-296
@@ -34,14 +37,22 @@ namespace vcg {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Orders this wave's LDS accesses across lanes.  A wave's LDS instructions
-// execute in issue order; this only keeps the compiler from moving LDS
-// accesses across the point.
-__device__ __forceinline__ void wsync()
+// Orders this wave's accesses to its node storage across lanes.  In LDS a
+// wave's DS instructions execute in issue order, so only the compiler has to
+// be held back; in HBM mode a workgroup-scope fence also waits for the
+// wave's stores before later loads.
+template <bool HBM>
+__device__ __forceinline__ void nsync()
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if constexpr (HBM) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -150,29 +161,43 @@ __device__ __forceinline__ bool past_deadline(const SolveArgs &a)
 // one worker's view of its current tree node
 // ---------------------------------------------------------------------------
 
+template <typename DegT, bool HBM>
 struct Node {
-    int32_t *deg;        // [npad] LDS: degree, -1 = in cover
-    uint32_t *mark;      // [mwords] LDS bitset, all zero between passes
-    uint32_t *list;      // [npad] LDS: vertices marked in the current pass
+    DegT *deg;           // [npad]: degree, -1 = in cover
+    uint32_t *mark;      // [mwords] bitset, all zero between passes
+    uint32_t *list;      // [lcap]: vertices marked in the current pass
     const uint32_t *src; // graph srcPtr (HBM)
     const uint32_t *dst; // graph dst (HBM)
     uint32_t n;
+    uint32_t lcap;
+    using deg_t = DegT;
+
+    __device__ __forceinline__ void sync() const { nsync<HBM>(); }
 };
 
 // Appends cand (kNone = nothing) of every lane to the mark list; with dedupe
-// a vertex already marked in this pass is dropped.  Returns the new length
-// (wave-uniform).
-__device__ __forceinline__ uint32_t append(const Node &w, uint32_t L, uint32_t cand, bool dedupe)
+// a vertex already marked in this pass is dropped.  At most lcap entries are
+// kept: a candidate past the end is unmarked again and found by the next
+// pass (applying any subset of a pass's marks is sound, since the whole set
+// is).  Returns the new length (wave-uniform).
+template <class N>
+__device__ __forceinline__ uint32_t append(const N &w, uint32_t L, uint32_t cand, bool dedupe)
 {
     bool keep = cand != kNone;
+    uint32_t bit = 1u << (cand & 31u);
     if (dedupe && keep) {
-        uint32_t bit = 1u << (cand & 31u);
-        uint32_t old = atomicOr(&w.mark[cand >> 5], bit);
+        uint32_t old = __hip_atomic_fetch_or(&w.mark[cand >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         keep = (old & bit) == 0;
     }
     uint64_t m = __ballot(keep);
-    if (keep) w.list[L + lanes_below(m)] = cand;
-    return L + (uint32_t)__popcll(m);
+    if (keep) {
+        uint32_t pos = L + lanes_below(m);
+        if (pos < w.lcap)
+            w.list[pos] = cand;
+        else if (dedupe)
+            __hip_atomic_fetch_and(&w.mark[cand >> 5], ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+    return min(L + (uint32_t)__popcll(m), w.lcap);
 }
 
 // Apply phase of a rule: every listed vertex goes into the cover.  First all
@@ -180,11 +205,12 @@ __device__ __forceinline__ uint32_t append(const Node &w, uint32_t L, uint32_t c
 // live (src/helperFunctions.cuh:270-289).  Listed vertices are distinct and
 // live.  Neighbours of one vertex are distinct, so one vertex's decrements
 // need no atomics; vertices are taken one after the other.
-__device__ void apply_list(const Node &w, uint32_t L)
+template <class N>
+__device__ void apply_list(const N &w, uint32_t L)
 {
-    wsync();
+    w.sync();
     for (uint32_t i = lane_id(); i < L; i += kWave) w.deg[w.list[i]] = -1;
-    wsync();
+    w.sync();
     for (uint32_t c0 = 0; c0 < L; c0 += kWave) {
         uint32_t cnt = min(L - c0, (uint32_t)kWave);
         uint32_t my_b = 0, my_e = 0;
@@ -198,20 +224,21 @@ __device__ void apply_list(const Node &w, uint32_t L)
             uint32_t e = __builtin_amdgcn_readlane(my_e, j);
             for (uint32_t x = b + lane_id(); x < e; x += kWave) {
                 uint32_t t = w.dst[x];
-                int32_t dt = w.deg[t];
-                if (dt != -1) w.deg[t] = dt - 1;
+                int dt = w.deg[t];
+                if (dt != -1) w.deg[t] = (typename N::deg_t)(dt - 1);
             }
-            wsync();
+            w.sync();
         }
     }
     for (uint32_t i = lane_id(); i < L; i += kWave) w.mark[w.list[i] >> 5] = 0u;
-    wsync();
+    w.sync();
 }
 
 // Leaf rule (src/helperFunctions.cuh:229-296): for every degree-1 vertex v
 // mark its live neighbour u when deg(u) > 1, or deg(u) == 1 and u < v (so an
 // isolated edge loses exactly one end).  Repeat until nothing is marked.
-__device__ uint32_t rule_leaf(const Node &w)
+template <class N>
+__device__ uint32_t rule_leaf(const N &w)
 {
     uint32_t total = 0;
     for (;;) {
@@ -223,7 +250,7 @@ __device__ uint32_t rule_leaf(const Node &w)
                 uint32_t e = w.src[v + 1];
                 for (uint32_t x = w.src[v]; x < e; ++x) {
                     uint32_t u = w.dst[x];
-                    int32_t du = w.deg[u];
+                    int du = w.deg[u];
                     if (du > 0) {
                         if (du > 1 || u < v) cand = u;
                         break;
@@ -259,7 +286,8 @@ __device__ __forceinline__ bool adj_has(const uint32_t *dst, uint32_t lo, uint32
 // vertex whose two live neighbours are adjacent puts both neighbours in the
 // cover.  Skipped when a neighbour has degree 1, or degree 2 with a lower
 // index, as in the reference.  All triangles of a pass are applied together.
-__device__ uint32_t rule_triangle(const Node &w)
+template <class N>
+__device__ uint32_t rule_triangle(const N &w)
 {
     uint32_t total = 0;
     for (;;) {
@@ -273,7 +301,7 @@ __device__ uint32_t rule_triangle(const Node &w)
                 uint32_t e = w.src[v + 1];
                 for (uint32_t x = w.src[v]; x < e; ++x) {
                     uint32_t u = w.dst[x];
-                    int32_t du = w.deg[u];
+                    int du = w.deg[u];
                     if (du > 0) {
                         if (du == 1 || (du == 2 && u < v)) {
                             keep = true;
@@ -304,7 +332,8 @@ __device__ uint32_t rule_triangle(const Node &w)
 
 // High-degree rule (src/helperFunctions.cuh:299-358): v goes into the cover
 // when deg(v) + d + (deleted so far in this call) >= T.
-__device__ uint32_t rule_high(const Node &w, uint32_t d, uint32_t T)
+template <class N>
+__device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T)
 {
     uint32_t total = 0;
     for (;;) {
@@ -313,7 +342,7 @@ __device__ uint32_t rule_high(const Node &w, uint32_t d, uint32_t T)
             uint32_t v = base + lane_id();
             uint32_t cand = kNone;
             if (v < w.n) {
-                int32_t dv = w.deg[v];
+                int dv = w.deg[v];
                 if (dv > 0 && (uint32_t)dv + d + total >= T) cand = v;
             }
             L = append(w, L, cand, false);
@@ -327,7 +356,8 @@ __device__ uint32_t rule_high(const Node &w, uint32_t d, uint32_t T)
 
 // Rule fixpoint in the order of src/GlobalWorkList.cuh:107-130.  Returns the
 // new number of vertices in the cover.
-__device__ uint32_t reduce_node(const Node &w, uint32_t d, uint32_t T)
+template <class N>
+__device__ uint32_t reduce_node(const N &w, uint32_t d, uint32_t T)
 {
     uint32_t it = 0, nl, nt = 0, nh = 0;
     do {
@@ -356,14 +386,15 @@ struct Summary {
     uint32_t maxv;   // lowest index of maximum degree
 };
 
-__device__ Summary summarize(const Node &w)
+template <class N>
+__device__ Summary summarize(const N &w)
 {
-    int32_t s = 0, md = 0;
+    int s = 0, md = 0;
     uint32_t mv = kNone;
     for (uint32_t base = 0; base < w.n; base += kWave) {
         uint32_t v = base + lane_id();
         if (v < w.n) {
-            int32_t dv = w.deg[v];
+            int dv = w.deg[v];
             if (dv > 0) s += dv;
             if (dv > md) {
                 md = dv;
@@ -381,53 +412,63 @@ __device__ Summary summarize(const Node &w)
 
 // Left child in place: v into the cover (deleteMaxDegreeVertex,
 // src/helperFunctions.cuh:200-215).
-__device__ void take_vertex(const Node &w, uint32_t v)
+template <class N>
+__device__ void take_vertex(const N &w, uint32_t v)
 {
-    wsync();
+    w.sync();
     if (lane_id() == 0) w.deg[v] = -1;
-    wsync();
+    w.sync();
     uint32_t e = w.src[v + 1];
     for (uint32_t x = w.src[v] + lane_id(); x < e; x += kWave) {
         uint32_t t = w.dst[x];
-        int32_t dt = w.deg[t];
-        if (dt != -1) w.deg[t] = dt - 1;
+        int dt = w.deg[t];
+        if (dt != -1) w.deg[t] = (typename N::deg_t)(dt - 1);
     }
-    wsync();
+    w.sync();
 }
 
 // Right child: every live neighbour of v into the cover
-// (deleteNeighborsOfMaxDegreeVertex, src/helperFunctions.cuh:156-184).
-// Returns how many vertices were added (deg(v) before the call).
-__device__ uint32_t take_neighbours(const Node &w, uint32_t v)
+// (deleteNeighborsOfMaxDegreeVertex, src/helperFunctions.cuh:156-184), in
+// rounds of at most lcap vertices.  Returns how many vertices were added
+// (deg(v) before the call).
+template <class N>
+__device__ uint32_t take_neighbours(const N &w, uint32_t v)
 {
-    uint32_t L = 0;
-    uint32_t b = w.src[v], e = w.src[v + 1];
-    for (uint32_t x0 = b; x0 < e; x0 += kWave) {
-        uint32_t x = x0 + lane_id();
-        uint32_t cand = kNone;
-        if (x < e) {
-            uint32_t t = w.dst[x];
-            if (w.deg[t] != -1) cand = t;
+    uint32_t total = 0;
+    const uint32_t b = w.src[v], e = w.src[v + 1];
+    for (;;) {
+        uint32_t L = 0;
+        for (uint32_t x0 = b; x0 < e && L < w.lcap; x0 += kWave) {
+            uint32_t x = x0 + lane_id();
+            uint32_t cand = kNone;
+            if (x < e) {
+                uint32_t t = w.dst[x];
+                if (w.deg[t] != -1) cand = t;
+            }
+            L = append(w, L, cand, false);
         }
-        L = append(w, L, cand, false);
+        if (L == 0) break;
+        apply_list(w, L);
+        total += L;
     }
-    if (L) apply_list(w, L);
-    return L;
+    return total;
 }
 
-// LDS <-> HBM copies of one node's degree array (npad ints, npad % 16 == 0).
-__device__ __forceinline__ void copy_to_hbm(int32_t *__restrict__ g, const int32_t *s, uint32_t npad)
+// Node storage <-> HBM slot copies (bytes % 16 == 0, both sides 16-aligned).
+template <bool HBM>
+__device__ __forceinline__ void copy_out(void *__restrict__ g, const void *s, uint32_t bytes)
 {
     const int4 *s4 = reinterpret_cast<const int4 *>(s);
     int4 *g4 = reinterpret_cast<int4 *>(g);
-    for (uint32_t i = lane_id(); i < npad / 4; i += kWave) g4[i] = s4[i];
+    for (uint32_t i = lane_id(); i < bytes / 16; i += kWave) g4[i] = s4[i];
 }
-__device__ __forceinline__ void copy_from_hbm(int32_t *s, const int32_t *__restrict__ g, uint32_t npad)
+template <bool HBM>
+__device__ __forceinline__ void copy_in(void *s, const void *__restrict__ g, uint32_t bytes)
 {
     const int4 *g4 = reinterpret_cast<const int4 *>(g);
     int4 *s4 = reinterpret_cast<int4 *>(s);
-    for (uint32_t i = lane_id(); i < npad / 4; i += kWave) s4[i] = g4[i];
-    wsync();
+    for (uint32_t i = lane_id(); i < bytes / 16; i += kWave) s4[i] = g4[i];
+    nsync<HBM>();
 }
 
 // ---------------------------------------------------------------------------
@@ -436,7 +477,8 @@ __device__ __forceinline__ void copy_from_hbm(int32_t *s, const int32_t *__restr
 
 // Writes (g, d, v, deg) into the ring if the occupancy guard admits it.
 // The caller has already checked the donation threshold.
-__device__ bool try_enqueue(const SolveArgs &a, const Node &w, uint32_t g, uint32_t d, uint32_t v)
+template <class N>
+__device__ bool try_enqueue(const SolveArgs &a, const N &w, uint32_t g, uint32_t d, uint32_t v, uint32_t slot_bytes)
 {
     uint32_t ok = 0, slot = 0, ticket = 0;
     if (lane_id() == 0) {
@@ -463,7 +505,7 @@ __device__ bool try_enqueue(const SolveArgs &a, const Node &w, uint32_t g, uint3
     slot = uni(slot);
     ticket = uni(ticket);
     fence_acq();
-    copy_to_hbm(a.q_deg + (size_t)slot * a.npad, w.deg, a.npad);
+    copy_out<false>(static_cast<char *>(a.q_deg) + (size_t)slot * slot_bytes, w.deg, slot_bytes);
     if (lane_id() == 0) a.q_meta[slot] = make_uint4(g, d, v, 0u);
     fence_rel();
     if (lane_id() == 0) {
@@ -474,8 +516,9 @@ __device__ bool try_enqueue(const SolveArgs &a, const Node &w, uint32_t g, uint3
 }
 
 // Takes one item if the occupancy guard says one is there.  On success the
-// degree array is in LDS and (g, d, v) are returned.
-__device__ bool try_dequeue(const SolveArgs &a, const Node &w, uint32_t &g, uint32_t &d, uint32_t &v)
+// degree array is in the worker's node storage and (g, d, v) are returned.
+template <class N>
+__device__ bool try_dequeue(const SolveArgs &a, const N &w, uint32_t &g, uint32_t &d, uint32_t &v, uint32_t slot_bytes)
 {
     uint32_t ok = 0, slot = 0, ticket = 0;
     if (lane_id() == 0 && ld_rlx(&a.glob->count) > 0) {
@@ -500,7 +543,8 @@ __device__ bool try_dequeue(const SolveArgs &a, const Node &w, uint32_t &g, uint
     slot = uni(slot);
     ticket = uni(ticket);
     fence_acq();
-    copy_from_hbm(w.deg, a.q_deg + (size_t)slot * a.npad, a.npad);
+    copy_in<false>(w.deg, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, slot_bytes);
+    w.sync();
     uint4 meta = a.q_meta[slot];
     g = uni(meta.x);
     d = uni(meta.y);
@@ -518,7 +562,8 @@ __device__ bool try_dequeue(const SolveArgs &a, const Node &w, uint32_t &g, uint
 // per-graph bookkeeping
 // ---------------------------------------------------------------------------
 
-__device__ void write_cover(const SolveArgs &a, const Node &w, const GDesc &gd)
+template <class N>
+__device__ void write_cover(const SolveArgs &a, const N &w, const GDesc &gd)
 {
     uint32_t *cw = a.covers + gd.off_cover;
     for (uint32_t base = 0; base < w.n; base += kWave) {
@@ -532,7 +577,8 @@ __device__ void write_cover(const SolveArgs &a, const Node &w, const GDesc &gd)
 }
 
 // A cover of size d was reached (maxDegree == 0).
-__device__ void record_cover(const SolveArgs &a, const Node &w, const GDesc &gd, uint32_t g, uint32_t d)
+template <class N>
+__device__ void record_cover(const SolveArgs &a, const N &w, const GDesc &gd, uint32_t g, uint32_t d)
 {
     GState *gs = &a.gstate[g];
     if (gd.instance == 0) {
@@ -596,12 +642,13 @@ __device__ __forceinline__ void finish_unit(const SolveArgs &a, uint32_t g)
 }
 
 // Idle worker: take a queued item, else a new root, else wait; returns
-// 0 = terminate, 1 = queued item (degrees already in LDS), 2 = root index.
+// 0 = terminate, 1 = queued item (degrees already in node storage), 2 = root.
 struct Acquired {
     uint32_t kind, g, d, v, root;
 };
 
-__device__ Acquired acquire(const SolveArgs &a, const Node &w)
+template <class N>
+__device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
 {
     Acquired r = {0u, kNone, 0u, kNone, 0u};
     if (lane_id() == 0) fetch_add(&a.glob->counter, kWaitOne);
@@ -609,7 +656,7 @@ __device__ Acquired acquire(const SolveArgs &a, const Node &w)
     for (;;) {
         if (a.version == 1) {
             uint32_t g, d, v;
-            if (try_dequeue(a, w, g, d, v)) {
+            if (try_dequeue(a, w, g, d, v, slot_bytes)) {
                 r.kind = 1;
                 r.g = g;
                 r.d = d;
@@ -655,25 +702,30 @@ __device__ __forceinline__ long long sq64(int x) { return (long long)x * (long l
 
 constexpr uint32_t kFlushEvery = 1024;
 
-__global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
+template <typename DegT, bool HBM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void solve_kernel(SolveArgs a)
 {
-    extern __shared__ int32_t smem[];
+    extern __shared__ int4 smem4[];
     const uint32_t wib = threadIdx.x / kWave;
     const uint32_t wid = blockIdx.x * (blockDim.x / kWave) + wib;
     if (wid >= a.workers) return;
 
-    const uint32_t per_worker = 2u * a.npad + a.mwords;
-    Node w;
-    w.deg = smem + wib * per_worker;
-    w.list = reinterpret_cast<uint32_t *>(w.deg + a.npad);
-    w.mark = w.list + a.npad;
+    // node storage: deg [npad DegT] | list [lcap u32] | mark [mwords u32]
+    char *store = HBM ? a.hbm_nodes + (size_t)wid * a.worker_bytes
+                      : reinterpret_cast<char *>(smem4) + (size_t)wib * a.worker_bytes;
+    const uint32_t slot_bytes = a.npad * (uint32_t)sizeof(DegT);
+    Node<DegT, HBM> w;
+    w.deg = reinterpret_cast<DegT *>(store);
+    w.list = reinterpret_cast<uint32_t *>(store + slot_bytes);
+    w.mark = w.list + a.lcap;
     w.src = nullptr;
     w.dst = nullptr;
     w.n = 0;
+    w.lcap = a.lcap;
     for (uint32_t i = lane_id(); i < a.mwords; i += kWave) w.mark[i] = 0u;
-    wsync();
+    w.sync();
 
-    int32_t *stk = a.stk + (size_t)wid * a.depth * a.npad;
+    char *stk = static_cast<char *>(a.stk) + (size_t)wid * a.depth * slot_bytes;
     uint2 *stk_meta = a.stk_meta + (size_t)wid * a.depth;
 
     if (lane_id() == 0) fetch_add(&a.glob->counter, kRegOne);
@@ -691,7 +743,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
             if (top >= 0) {
                 // pop (popStack, src/stack.cuh:10-20), then build the stored
                 // right child
-                copy_from_hbm(w.deg, stk + (size_t)top * a.npad, a.npad);
+                copy_in<HBM>(w.deg, stk + (size_t)top * slot_bytes, slot_bytes);
                 uint2 m = stk_meta[top];
                 d = uni(m.x);
                 uint32_t v = uni(m.y);
@@ -706,7 +758,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                     g = kNone;
                 }
                 if (lane_id() == 0 && ld_rlx(&a.glob->cancel)) st_rlx(&a.glob->done_flag, 1u);
-                Acquired q = acquire(a, w);
+                Acquired q = acquire(a, w, slot_bytes);
                 if (q.kind == 0) break;
                 if (q.kind == 1) {
                     g = q.g;
@@ -723,8 +775,9 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                     descend = 0;
                 } else {
                     const int32_t *deg0 = reinterpret_cast<const int32_t *>(a.gbuf + gd.off_deg);
-                    for (uint32_t i = lane_id(); i < a.npad; i += kWave) w.deg[i] = i < gd.n ? deg0[i] : -1;
-                    wsync();
+                    for (uint32_t i = lane_id(); i < a.npad; i += kWave)
+                        w.deg[i] = (DegT)(i < gd.n ? deg0[i] : -1);
+                    w.sync();
                     d = 0;
                     if (a.version == 1) {
                         descend = 0;
@@ -797,7 +850,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
             unsigned long long c = 0;
             if (lane_id() == 0) c = ld_rlx(&a.glob->counter);
             int enq = (int)uni((uint32_t)(c >> 32));
-            if (enq < (int)a.qthreshold) donated = try_enqueue(a, w, g, d, v);
+            if (enq < (int)a.qthreshold) donated = try_enqueue(a, w, g, d, v, slot_bytes);
         }
         if (!donated) {
             if (top + 1 >= (int)a.depth) {
@@ -806,7 +859,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                 continue;
             }
             ++top;
-            copy_to_hbm(stk + (size_t)top * a.npad, w.deg, a.npad);
+            copy_out<HBM>(stk + (size_t)top * slot_bytes, w.deg, slot_bytes);
             if (lane_id() == 0) stk_meta[top] = make_uint2(d, v);
         }
         take_vertex(w, v);
@@ -837,6 +890,10 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
         *a.glob = z;
     }
 }
+
+template __global__ void solve_kernel<int16_t, false>(SolveArgs);
+template __global__ void solve_kernel<int32_t, false>(SolveArgs);
+template __global__ void solve_kernel<int32_t, true>(SolveArgs);
 
 }  // namespace vcg
 
@@ -916,32 +973,68 @@ bool csr_is_valid(const vc_csr &g)
     return true;
 }
 
+// Kernel variants: degree type x node storage.
+enum Variant { kLds16 = 0, kLds32 = 1, kHbm32 = 2 };
+
+const void *kernel_of(int variant)
+{
+    switch (variant) {
+    case kLds16: return reinterpret_cast<const void *>(vcg::solve_kernel<int16_t, false>);
+    case kLds32: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, false>);
+    default: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, true>);
+    }
+}
+
 struct Plan {
-    uint32_t npad, mwords, wpb, block_dim, blocks_per_cu, num_blocks, workers;
+    int variant;
+    uint32_t npad, lcap, mwords, worker_bytes, deg_bytes, wpb, block_dim, blocks_per_cu, num_blocks, workers;
     uint32_t depth, qsize, qthreshold, roots, units;
     size_t smem;
-    size_t off_glob, off_stk, off_meta, off_qdeg, off_qmeta, off_tickets, off_wnodes, total;
+    size_t off_glob, off_stk, off_meta, off_qdeg, off_qmeta, off_tickets, off_wnodes, off_hbm, total;
 };
+
+// Per-worker node storage for a degree type: deg [npad] | list [lcap] | mark [mwords].
+void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes)
+{
+    pl.deg_bytes = deg_bytes;
+    pl.npad = round_up(nmax, 64u / deg_bytes);
+    pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), 1024u);
+    pl.mwords = round_up((nmax + 31u) / 32u, 4u);
+    pl.worker_bytes = round_up(pl.npad * deg_bytes + 4u * pl.lcap + 4u * pl.mwords, 64u);
+}
 
 vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *p, Plan &pl)
 {
     const uint32_t version = p->version;
     if (version != VC_HYBRID && version != VC_STACK_ONLY) return VC_EINVAL;
-    if (p->use_global_memory == 1) return VC_ENOTSUP;  // HBM-resident degree arrays: not built yet
+    if (p->use_global_memory < -1 || p->use_global_memory > 1) return VC_EINVAL;
     uint32_t block_dim = p->block_dim ? p->block_dim : 256u;
     if (block_dim % vcg::kWave != 0 || block_dim > 256u) return VC_EINVAL;
     if (version == VC_STACK_ONLY && p->start_depth > 20u) return VC_EINVAL;
-    pl.block_dim = block_dim;
-    pl.wpb = block_dim / vcg::kWave;
     const uint32_t nmax = std::max<uint32_t>(set->nmax, 1u);
-    pl.npad = round_up(nmax, 16u);
-    pl.mwords = round_up((nmax + 31u) / 32u, 4u);
-    pl.smem = (size_t)pl.wpb * (2u * pl.npad + pl.mwords) * sizeof(int32_t);
-    if (pl.smem > 160u * 1024u) return VC_ENOTSUP;
+    const size_t lds_limit = ctx->lds_per_cu ? ctx->lds_per_cu : 160u * 1024u;
+
+    // Degree arrays in LDS when at least one worker fits per CU (int16 while
+    // every degree fits), otherwise in HBM (-g 1 forces HBM, -g 0 forbids it).
+    pl.variant = nmax < 32768u ? kLds16 : kLds32;
+    size_storage(pl, nmax, pl.variant == kLds16 ? 2u : 4u);
+    bool hbm = p->use_global_memory == 1 || pl.worker_bytes > lds_limit;
+    if (hbm && p->use_global_memory == 0) return VC_ENOTSUP;
+    if (hbm) {
+        pl.variant = kHbm32;
+        size_storage(pl, nmax, 4u);
+    }
+    // a worker is one wave: shrink the workgroup when LDS would not hold it
+    uint32_t wpb = block_dim / vcg::kWave;
+    if (!hbm && !p->block_dim)
+        while (wpb > 1 && (size_t)wpb * pl.worker_bytes > lds_limit) wpb /= 2;
+    pl.wpb = wpb;
+    pl.block_dim = wpb * vcg::kWave;
+    pl.smem = hbm ? 0 : (size_t)pl.wpb * pl.worker_bytes;
+    if (pl.smem > lds_limit) return VC_ENOTSUP;
 
     int bpc = 0;
-    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, reinterpret_cast<const void *>(vcg::solve_kernel),
-                                                             (int)block_dim, pl.smem)))
+    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel_of(pl.variant), (int)pl.block_dim, pl.smem)))
         return VC_EDEVICE;
     if (bpc < 1) return VC_ENOTSUP;
     pl.blocks_per_cu = (uint32_t)bpc;
@@ -974,7 +1067,7 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
 
     // Fit stacks + worklist into device memory; shrink the grid if needed
     // (the reference halves the worklist instead, src/auxFunctions.cpp:507-512).
-    const size_t slot = (size_t)pl.npad * sizeof(int32_t);
+    const size_t slot = (size_t)pl.npad * pl.deg_bytes;
     size_t budget = ctx->total_mem / 10 * 7;
     for (;;) {
         pl.workers = pl.num_blocks * pl.wpb;
@@ -991,6 +1084,7 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
         pl.off_qmeta = take((size_t)pl.qsize * sizeof(uint4));
         pl.off_tickets = take((size_t)pl.qsize * sizeof(uint32_t));
         pl.off_wnodes = take((size_t)pl.workers * sizeof(uint64_t));
+        pl.off_hbm = take(pl.variant == kHbm32 ? (size_t)pl.workers * pl.worker_bytes : 0);
         pl.total = off;
         if (pl.total <= budget || pl.num_blocks <= 1) break;
         pl.num_blocks = std::max<uint32_t>(1u, pl.num_blocks / 2u);
@@ -1149,7 +1243,7 @@ vc_status vc_occupancy(vc_ctx *ctx, const vc_graphs *set, const vc_params *param
     if (st != VC_OK) return st;
     if (blocks_per_cu) *blocks_per_cu = (int)pl.blocks_per_cu;
     if (block_dim) *block_dim = (int)pl.block_dim;
-    if (use_global_memory) *use_global_memory = 0;
+    if (use_global_memory) *use_global_memory = pl.variant == kHbm32;
     return VC_OK;
 }
 
@@ -1200,15 +1294,17 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     a.gstate = set->d_state;
     a.covers = set->d_covers;
     a.glob = reinterpret_cast<Global *>(base + pl.off_glob);
-    a.stk = reinterpret_cast<int32_t *>(base + pl.off_stk);
+    a.stk = base + pl.off_stk;
     a.stk_meta = reinterpret_cast<uint2 *>(base + pl.off_meta);
-    a.q_deg = reinterpret_cast<int32_t *>(base + pl.off_qdeg);
+    a.q_deg = base + pl.off_qdeg;
     a.q_meta = reinterpret_cast<uint4 *>(base + pl.off_qmeta);
     a.tickets = reinterpret_cast<uint32_t *>(base + pl.off_tickets);
     a.worker_nodes = reinterpret_cast<uint64_t *>(base + pl.off_wnodes);
     a.num_graphs = (uint32_t)set->count;
     a.npad = pl.npad;
+    a.lcap = pl.lcap;
     a.mwords = pl.mwords;
+    a.worker_bytes = pl.worker_bytes;
     a.depth = pl.depth;
     a.qsize = pl.qsize;
     a.qthreshold = pl.qthreshold;
@@ -1221,13 +1317,25 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         uint64_t ms = params->time_limit_ms ? params->time_limit_ms : 3600ull * 1000ull;
         a.limit_ticks = ms * (uint64_t)ctx->wall_khz;
     }
-    a.hbm_mode = 0;
-    a.hbm_scratch = nullptr;
+    a.hbm_nodes = pl.variant == kHbm32 ? base + pl.off_hbm : nullptr;
 
     hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, ctx->stream, a, pl.units);
     if (!hip_ok(hipGetLastError())) return VC_EDEVICE;
     hipEventRecord(ctx->ev0, ctx->stream);
-    hipLaunchKernelGGL(vcg::solve_kernel, dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem, ctx->stream, a);
+    switch (pl.variant) {
+    case kLds16:
+        hipLaunchKernelGGL((vcg::solve_kernel<int16_t, false>), dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem,
+                           ctx->stream, a);
+        break;
+    case kLds32:
+        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, false>), dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem,
+                           ctx->stream, a);
+        break;
+    default:
+        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, true>), dim3(pl.num_blocks), dim3(pl.block_dim), 0,
+                           ctx->stream, a);
+        break;
+    }
     if (!hip_ok(hipGetLastError())) return VC_EDEVICE;
     hipEventRecord(ctx->ev1, ctx->stream);
 
@@ -1283,7 +1391,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         info->block_dim = pl.block_dim;
         info->workers = pl.workers;
         info->blocks_per_cu = pl.blocks_per_cu;
-        info->use_global_memory = 0;
+        info->use_global_memory = pl.variant == kHbm32;
         info->list_size = pl.qsize;
         info->list_threshold = pl.qthreshold;
         info->stack_depth = pl.depth;

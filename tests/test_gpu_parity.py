@@ -147,3 +147,60 @@ def test_medium_graphs_match_oracle(ctx, version):
         assert mvc == want, (name, mvc, want)
         assert O.cover_is_valid(og, cover), name
         assert int(np.sum(cover)) == want, name
+
+
+def test_hbm_mode_matches_oracle(ctx):
+    """Degree arrays in HBM (the reference's _global_ kernels, -g 1)."""
+    for name, n, e in _medium_cases()[:8]:
+        og = O.from_edges(n, e)
+        want, _, _ = O.mvc(og)
+        g = vcgpu.csr_from_edges(n, e)
+        mvc, cover, r, info = vcgpu.solve_mvc(ctx, g, P(use_global_memory=1))
+        assert info.use_global_memory == 1
+        assert r.solved and mvc == want, (name, mvc, want)
+        assert O.cover_is_valid(og, cover) and int(np.sum(cover)) == want
+
+
+def _known():
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    from golden.make_golden import unpack_graph
+    meta = json.load(open(os.path.join(here, "golden", "known_answers.json")))
+    out = []
+    for name, m in sorted(meta.items()):
+        n, e = unpack_graph(os.path.join(here, "golden", "graphs", name + ".npz"))
+        out.append((name, n, e, m))
+    return out
+
+
+@pytest.mark.parametrize("case", [k for k in _known() if k[3]["cpu_confirmed"]], ids=lambda k: k[0])
+def test_reference_known_answers(ctx, case):
+    """The reference's own MVC table (test/runGraphs.py:8-30) on its graph files."""
+    name, n, e, meta = case
+    g = vcgpu.csr_from_edges(n, e)
+    og = O.from_edges(n, e)
+    mvc, cover, r, info = vcgpu.solve_mvc(ctx, g, P())
+    assert r.solved, name
+    assert mvc == meta["mvc"], (name, mvc, meta["mvc"])
+    assert O.cover_is_valid(og, np.asarray(cover, np.uint8)) and int(np.sum(cover)) == mvc
+
+
+@pytest.mark.parametrize("case", [k for k in _known() if not k[3]["cpu_confirmed"]], ids=lambda k: k[0])
+def test_reference_known_answers_hard(ctx, case):
+    """Graphs of the same table the CPU oracle cannot finish in minutes: with
+    a 30 s limit the device incumbent must never go below the known MVC, its
+    cover must be valid, and a finished search must hit the known value."""
+    name, n, e, meta = case
+    g = vcgpu.csr_from_edges(n, e)
+    og = O.from_edges(n, e)
+    U, ucov = vcgpu.upper_bound(g, seed=1)
+    gs = ctx.upload(g)
+    (r,), info, st = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), P(time_limit_ms=30000), raise_on_error=False)
+    assert r.minimum >= meta["mvc"], (name, r.minimum)
+    cover = r.cover if r.cover is not None else ucov
+    assert O.cover_is_valid(og, np.asarray(cover, np.uint8)) and int(np.sum(cover)) == r.minimum
+    if r.solved:
+        assert r.minimum == meta["mvc"]
+    print(f"{name}: solved={r.solved} minimum={r.minimum} known={meta['mvc']} nodes={r.tree_nodes} "
+          f"kernel_ms={info.kernel_ms:.0f}")
