@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--block-dim", type=int, default=0)
     ap.add_argument("--num-blocks", type=int, default=0)
     ap.add_argument("--time-limit-ms", type=int, default=120000)
+    ap.add_argument("--list-size", type=int, default=0, help="worklist slots (-q), 0 = auto")
+    ap.add_argument("--list-threshold", type=float, default=0.0, help="donation threshold fraction (-t), 0 = auto")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -118,7 +120,8 @@ def main():
 
     ctx = vcgpu.Context(0)
     params = vcgpu.Params(version=vcgpu.VC_HYBRID, block_dim=args.block_dim, num_blocks=args.num_blocks,
-                          time_limit_ms=args.time_limit_ms)
+                          time_limit_ms=args.time_limit_ms, list_size=args.list_size,
+                          list_threshold=args.list_threshold)
     out = {"metric": "graphs/sec and tree-nodes/sec at exact MVC; p99 latency (batched vs plain)",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded generators)"}
@@ -152,7 +155,7 @@ def main():
             (r,), _, _ = ctx.solve(one, jobs[i], params, want_covers=False, raise_on_error=False)
             plain_solved += int(r.solved)
             one.free()
-        plain_wall = time.perf_counter() - t0
+        plain_wall = max(time.perf_counter() - t0, 1e-9)
         out.update({
             "value": solved / wall, "unit": "graphs/s", "ms_per_step": wall * 1000.0,
             "config": {"workload": "batched MVC, 4096 graphs n~U[40,400], half ER p~U[0.02,0.15], half BA "
@@ -162,6 +165,7 @@ def main():
                        "tree_nodes_per_s": nodes / wall, "kernel_ms": kernel_s * 1000.0,
                        "workers": info.workers, "block_dim": info.block_dim, "num_blocks": info.num_blocks,
                        "blocks_per_cu": info.blocks_per_cu, "status": st,
+                       "idle_fraction": info.idle_fraction, "donations": info.donations, "steals": info.steals,
                        "plain_graphs_per_s": plain_solved / plain_wall, "plain_sample": ps,
                        "batched_over_plain": (solved / wall) / max(plain_solved / plain_wall, 1e-12)},
             "roofline": {"bound": "hbm", "achieved": alg_bytes / kernel_s / 1e9, "peak": HBM_PEAK_GBS,
@@ -204,7 +208,8 @@ def main():
             "value": len(jobs_list) / wall, "unit": "graphs/s", "ms_per_step": wall * 1000.0,
             "config": {"workload": args.workload, "n": n, "m": len(e), "U": U, "answers": answers[-len(jobs_list):],
                        "tree_nodes_per_step": nodes, "tree_nodes_per_s": nodes / wall,
-                       "kernel_ms": kernel_s * 1000.0, "workers": info.workers},
+                       "kernel_ms": kernel_s * 1000.0, "workers": info.workers,
+                       "idle_fraction": info.idle_fraction, "donations": info.donations, "steals": info.steals},
             "roofline": {"bound": "hbm", "achieved": nodes * b_node(n, len(e)) / kernel_s / 1e9,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": nodes * b_node(n, len(e)) / kernel_s / 1e9 / HBM_PEAK_GBS, "traffic": None},

@@ -107,6 +107,10 @@ typedef struct {
     float kernel_ms;
     uint32_t cu_count;
     uint64_t tree_nodes;
+    uint64_t donations;   /* subtrees put on the worklist */
+    uint64_t steals;      /* subtrees taken from the worklist */
+    float idle_fraction;  /* share of worker time spent waiting for work */
+    uint32_t reserved;
 } vc_launch_info;
 
 typedef struct vc_ctx vc_ctx;

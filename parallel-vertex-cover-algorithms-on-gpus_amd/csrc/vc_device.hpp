@@ -73,6 +73,14 @@ constexpr unsigned long long kEnqOne = 1ull << 32;
 // glob->error codes
 constexpr uint32_t kErrStackOverflow = 5;
 
+// Per-worker counters written once when the worker exits.
+struct WorkerStats {
+    uint64_t nodes;       // tree nodes processed
+    uint64_t idle_ticks;  // device wall-clock ticks spent waiting for work
+    uint64_t donations;   // subtrees put on the worklist
+    uint64_t steals;      // subtrees taken from the worklist
+};
+
 // Degree arrays are stored as DegT (int16 when every graph of the launch
 // has fewer than 32768 vertices, else int32); a node slot on a stack or in
 // the worklist is npad DegT values, a multiple of 64 bytes.
@@ -87,7 +95,7 @@ struct SolveArgs {
     void *q_deg;            // [qsize][npad] DegT
     uint4 *q_meta;          // [qsize] = (graph, d, branch vertex, 0)
     uint32_t *tickets;      // [qsize]
-    uint64_t *worker_nodes; // [workers]
+    WorkerStats *wstats;    // [workers]
     uint32_t num_graphs;
     uint32_t npad;          // slot length in DegT values
     uint32_t lcap;          // mark-list capacity per worker (multiple of 4)

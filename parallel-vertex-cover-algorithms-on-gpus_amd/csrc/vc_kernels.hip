@@ -475,10 +475,11 @@ __device__ __forceinline__ void copy_in(void *s, const void *__restrict__ g, uin
 // broker worklist (src/BWDWorkList.cuh) with 64-bit ring positions
 // ---------------------------------------------------------------------------
 
-// Writes (g, d, v, deg) into the ring if the occupancy guard admits it.
-// The caller has already checked the donation threshold.
-template <class N>
-__device__ bool try_enqueue(const SolveArgs &a, const N &w, uint32_t g, uint32_t d, uint32_t v, uint32_t slot_bytes)
+// Writes one subtree (degrees at src, cover size d, pending branch vertex v
+// of graph g) into the ring if the occupancy guard admits it.  The caller
+// has already decided that a donation is wanted.
+__device__ bool try_enqueue(const SolveArgs &a, const void *src, uint32_t g, uint32_t d, uint32_t v,
+                            uint32_t slot_bytes)
 {
     uint32_t ok = 0, slot = 0, ticket = 0;
     if (lane_id() == 0) {
@@ -505,7 +506,7 @@ __device__ bool try_enqueue(const SolveArgs &a, const N &w, uint32_t g, uint32_t
     slot = uni(slot);
     ticket = uni(ticket);
     fence_acq();
-    copy_out<false>(static_cast<char *>(a.q_deg) + (size_t)slot * slot_bytes, w.deg, slot_bytes);
+    copy_out<false>(static_cast<char *>(a.q_deg) + (size_t)slot * slot_bytes, src, slot_bytes);
     if (lane_id() == 0) a.q_meta[slot] = make_uint4(g, d, v, 0u);
     fence_rel();
     if (lane_id() == 0) {
@@ -732,14 +733,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 
     uint32_t g = kNone;
     GDesc gd = {};
-    int top = -1;
+    int top = -1, bot = 0;  // local stack entries bot..top (bot moves up on donation)
     bool have = false;
     uint32_t d = 0;
     uint32_t path = 0, descend = 0;
-    uint64_t pending = 0, total_nodes = 0;
+    uint64_t pending = 0, total_nodes = 0, idle_ticks = 0, donations = 0, steals = 0;
 
     for (;;) {
         if (!have) {
+            if (top < bot) {
+                top = -1;
+                bot = 0;
+            }
             if (top >= 0) {
                 // pop (popStack, src/stack.cuh:10-20), then build the stored
                 // right child
@@ -758,8 +763,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                     g = kNone;
                 }
                 if (lane_id() == 0 && ld_rlx(&a.glob->cancel)) st_rlx(&a.glob->done_flag, 1u);
+                const uint64_t t_idle = wall_clock64();
                 Acquired q = acquire(a, w, slot_bytes);
+                idle_ticks += wall_clock64() - t_idle;
                 if (q.kind == 0) break;
+                steals += q.kind == 1;
                 if (q.kind == 1) {
                     g = q.g;
                 } else {
@@ -794,6 +802,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         if (uni(ld_rlx(&gs->stop))) {
             // graph finished by someone else (PVC found, budget): drop local work
             top = -1;
+            bot = 0;
             have = false;
             continue;
         }
@@ -804,6 +813,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             if (lane_id() == 0) stop_all = ld_rlx(&a.glob->cancel) || past_deadline(a);
             if (uni(stop_all)) {
                 top = -1;
+                bot = 0;
                 have = false;
                 continue;
             }
@@ -845,27 +855,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             }
             continue;
         }
-        bool donated = false;
+        // Branch: the right child (N(v) into the cover) is stored as its
+        // parent state plus v and built when it is popped.
+        if (top + 1 >= (int)a.depth) {
+            if (lane_id() == 0) st_rlx(&a.glob->error, kErrStackOverflow);
+            have = false;
+            continue;
+        }
+        ++top;
+        copy_out<HBM>(stk + (size_t)top * slot_bytes, w.deg, slot_bytes);
+        if (lane_id() == 0) stk_meta[top] = make_uint2(d, v);
         if (a.version == 1) {
+            // Donate while workers wait and the worklist holds fewer items
+            // than there are waiting workers (and fewer than the -t
+            // threshold).  The OLDEST local entry goes: it is the shallowest
+            // and so the largest subtree this worker holds.
             unsigned long long c = 0;
             if (lane_id() == 0) c = ld_rlx(&a.glob->counter);
+            int waiting = (int)(uni((uint32_t)c) & 0xFFFFu);
             int enq = (int)uni((uint32_t)(c >> 32));
-            if (enq < (int)a.qthreshold) donated = try_enqueue(a, w, g, d, v, slot_bytes);
-        }
-        if (!donated) {
-            if (top + 1 >= (int)a.depth) {
-                if (lane_id() == 0) st_rlx(&a.glob->error, kErrStackOverflow);
-                have = false;
-                continue;
+            if (waiting > 0 && enq < waiting && enq < (int)a.qthreshold && top >= bot) {
+                uint2 m = stk_meta[bot];
+                if (try_enqueue(a, stk + (size_t)bot * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
+                    ++bot;
+                    ++donations;
+                }
             }
-            ++top;
-            copy_out<HBM>(stk + (size_t)top * slot_bytes, w.deg, slot_bytes);
-            if (lane_id() == 0) stk_meta[top] = make_uint2(d, v);
         }
         take_vertex(w, v);
         d += 1;
     }
-    if (lane_id() == 0) a.worker_nodes[wid] = total_nodes;
+    if (lane_id() == 0) {
+        WorkerStats ws;
+        ws.nodes = total_nodes;
+        ws.idle_ticks = idle_ticks;
+        ws.donations = donations;
+        ws.steals = steals;
+        a.wstats[wid] = ws;
+    }
 }
 
 // Per-launch initialisation with plain stores (no memset nodes, so the whole
@@ -883,7 +910,7 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
         a.gstate[g] = s;
     }
     for (uint32_t i = tid; i < a.qsize; i += stride) a.tickets[i] = 0u;
-    for (uint32_t i = tid; i < a.workers; i += stride) a.worker_nodes[i] = 0ull;
+    for (uint32_t i = tid; i < a.workers; i += stride) a.wstats[i] = WorkerStats{};
     if (tid == 0) {
         Global z = {};
         z.t0 = wall_clock64();
@@ -1062,7 +1089,7 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
     if (p->list_threshold > 0.f) {
         pl.qthreshold = (uint32_t)(p->list_threshold * (float)pl.qsize);
     } else {
-        pl.qthreshold = std::min<uint32_t>(pl.qsize, std::max<uint32_t>(64u, pl.workers / 2u));
+        pl.qthreshold = pl.qsize;  // demand-driven: donate only while workers wait
     }
 
     // Fit stacks + worklist into device memory; shrink the grid if needed
@@ -1083,7 +1110,7 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
         pl.off_qdeg = take((size_t)pl.qsize * slot);
         pl.off_qmeta = take((size_t)pl.qsize * sizeof(uint4));
         pl.off_tickets = take((size_t)pl.qsize * sizeof(uint32_t));
-        pl.off_wnodes = take((size_t)pl.workers * sizeof(uint64_t));
+        pl.off_wnodes = take((size_t)pl.workers * sizeof(vcg::WorkerStats));
         pl.off_hbm = take(pl.variant == kHbm32 ? (size_t)pl.workers * pl.worker_bytes : 0);
         pl.total = off;
         if (pl.total <= budget || pl.num_blocks <= 1) break;
@@ -1299,7 +1326,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     a.q_deg = base + pl.off_qdeg;
     a.q_meta = reinterpret_cast<uint4 *>(base + pl.off_qmeta);
     a.tickets = reinterpret_cast<uint32_t *>(base + pl.off_tickets);
-    a.worker_nodes = reinterpret_cast<uint64_t *>(base + pl.off_wnodes);
+    a.wstats = reinterpret_cast<vcg::WorkerStats *>(base + pl.off_wnodes);
     a.num_graphs = (uint32_t)set->count;
     a.npad = pl.npad;
     a.lcap = pl.lcap;
@@ -1341,11 +1368,11 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
 
     std::vector<GState> hs(set->count);
     Global hg;
-    std::vector<uint64_t> wn(pl.workers);
+    std::vector<vcg::WorkerStats> wn(pl.workers);
     bool ok = hip_ok(hipMemcpyAsync(hs.data(), set->d_state, sizeof(GState) * set->count, hipMemcpyDeviceToHost,
                                     ctx->stream)) &&
               hip_ok(hipMemcpyAsync(&hg, a.glob, sizeof(Global), hipMemcpyDeviceToHost, ctx->stream)) &&
-              hip_ok(hipMemcpyAsync(wn.data(), a.worker_nodes, sizeof(uint64_t) * pl.workers, hipMemcpyDeviceToHost,
+              hip_ok(hipMemcpyAsync(wn.data(), a.wstats, sizeof(vcg::WorkerStats) * pl.workers, hipMemcpyDeviceToHost,
                                     ctx->stream));
     std::vector<uint32_t> bitmaps;
     bool want_covers = false;
@@ -1360,8 +1387,13 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     float ms = 0.f;
     hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
 
-    uint64_t total_nodes = 0;
-    for (uint64_t x : wn) total_nodes += x;
+    uint64_t total_nodes = 0, idle = 0, donations = 0, steals = 0;
+    for (const vcg::WorkerStats &x : wn) {
+        total_nodes += x.nodes;
+        idle += x.idle_ticks;
+        donations += x.donations;
+        steals += x.steals;
+    }
     vc_status overall = hg.error ? VC_EDEVICE : (hg.timed_out ? VC_ETIMEOUT : VC_OK);
     for (int i = 0; i < set->count; ++i) {
         const GState &s = hs[i];
@@ -1398,6 +1430,11 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         info->kernel_ms = ms;
         info->cu_count = (uint32_t)ctx->cu_count;
         info->tree_nodes = total_nodes;
+        info->donations = donations;
+        info->steals = steals;
+        // share of worker time spent waiting for work
+        double span = (double)ms * 1e-3 * (double)ctx->wall_khz * 1e3 * (double)pl.workers;
+        info->idle_fraction = span > 0 ? (float)std::min(1.0, (double)idle / span) : 0.f;
     }
     return overall;
 }

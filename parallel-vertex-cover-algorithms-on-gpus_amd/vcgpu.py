@@ -59,7 +59,8 @@ class _Info(C.Structure):
     _fields_ = [("num_blocks", C.c_uint32), ("block_dim", C.c_uint32), ("workers", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("use_global_memory", C.c_uint32), ("list_size", C.c_uint32),
                 ("list_threshold", C.c_uint32), ("stack_depth", C.c_uint32), ("kernel_ms", C.c_float),
-                ("cu_count", C.c_uint32), ("tree_nodes", C.c_uint64)]
+                ("cu_count", C.c_uint32), ("tree_nodes", C.c_uint64), ("donations", C.c_uint64),
+                ("steals", C.c_uint64), ("idle_fraction", C.c_float), ("reserved", C.c_uint32)]
 
 
 # Every symbol include/vcgpu.h declares.
@@ -217,6 +218,10 @@ class LaunchInfo:
     kernel_ms: float
     cu_count: int
     tree_nodes: int
+    donations: int
+    steals: int
+    idle_fraction: float
+    reserved: int
 
 
 @dataclass
