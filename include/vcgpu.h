@@ -29,8 +29,9 @@ typedef enum {
     VC_ETIMEOUT = 4,   /* node budget or spin bound exhausted before the search finished */
     VC_ECANCELLED = 5, /* cancelled by the caller */
     VC_ENOTSUP = 6,    /* combination not supported */
-    VC_EBADGRAPH = 7   /* duplicate edge / self-loop / index out of range
+    VC_EBADGRAPH = 7,  /* duplicate edge / self-loop / index out of range
                           (reference: assert(check_graph), src/auxFunctions.cpp:358) */
+    VC_EAGAIN = 8      /* serving: the request has not finished yet */
 } vc_status;
 
 /* src/config.h:125-129 */
@@ -143,6 +144,46 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const 
  * cudaOccupancyMaxActiveBlocksPerMultiprocessor, src/main.cu:131-191). */
 vc_status vc_occupancy(vc_ctx *ctx, const vc_graphs *set, const vc_params *params, int *blocks_per_cu,
                        int *block_dim, int *use_global_memory);
+
+/* ---- serving: continuous batching over one persistent launch ----
+ *
+ * No counterpart in the reference (which solves one graph per process); this
+ * is the request loop of the served configuration.  A server owns one
+ * persistent solve kernel on its own stream.  vc_server_submit copies a
+ * graph and returns a ticket at once; a host thread admits queued requests
+ * as request slots and arena memory (graph + cover + the worker-stack
+ * footprint check) free up, uploads them while the kernel runs, and
+ * publishes them to the kernel, whose idle workers pick them up.  Search
+ * state stays on the device; the kernel reports each completion to pinned
+ * host memory. */
+typedef struct vc_server vc_server;
+
+typedef struct {
+    uint32_t max_vertices;  /* largest graph the server admits (sizes LDS and stacks) */
+    uint32_t slots;         /* requests on the device at once; 0 = 4096 */
+    uint64_t arena_bytes;   /* HBM for the graphs + covers in flight; 0 = 1 GiB */
+    uint32_t num_blocks;    /* workgroups of the persistent kernel; 0 = occupancy x CUs */
+    uint32_t block_dim;     /* threads per workgroup (multiple of 64, <= 256); 0 = auto */
+    uint32_t stack_depth;   /* stack entries per worker; 0 = max_vertices + 1 */
+    uint32_t time_limit_ms; /* lifetime limit of the server kernel; 0 = one hour */
+} vc_server_config;
+
+vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **out);
+/* Queue one request (graph copied; U = job->upper_bound must be a valid
+ * cover size).  *ticket identifies it in poll/wait.  Graphs larger than
+ * max_vertices are rejected with VC_EINVAL. */
+vc_status vc_server_submit(vc_server *s, const vc_csr *graph, const vc_job *job, uint64_t *ticket);
+/* VC_EAGAIN while pending; otherwise fills *result (and cover, vertexNum
+ * bytes, if non-NULL) and forgets the ticket.  *done_ns (optional) is the
+ * host monotonic-clock time at which the host thread saw the completion. */
+vc_status vc_server_poll(vc_server *s, uint64_t ticket, vc_result *result, uint8_t *cover, uint64_t *done_ns);
+/* Blocks until the request finishes or timeout_ms passes (VC_EAGAIN). */
+vc_status vc_server_wait(vc_server *s, uint64_t ticket, uint32_t timeout_ms, vc_result *result, uint8_t *cover,
+                         uint64_t *done_ns);
+/* Host monotonic clock in ns (the clock of done_ns). */
+uint64_t vc_now_ns(void);
+/* Finishes every admitted and queued request, stops the kernel, frees all. */
+vc_status vc_server_stop(vc_server *s, vc_launch_info *info);
 
 /* ---- host helpers the reference's driver calls around the solve ---- */
 

@@ -20,11 +20,15 @@ struct GDesc {
     uint32_t instance;   // 0 MVC, 1 PVC
     uint32_t k;
     uint32_t U;          // upper bound (valid cover size)
-    uint32_t pad0;
+    uint32_t seq;        // serving: request sequence number echoed in the completion
     uint64_t max_nodes;  // 0 = unlimited
     uint32_t pad1[4];
 };
 static_assert(sizeof(GDesc) == 64, "GDesc is one 64-byte line");
+// the serving loader reads these fields by 32-bit word index
+static_assert(__builtin_offsetof(GDesc, n) == 0 && __builtin_offsetof(GDesc, off_src) == 8 &&
+                  __builtin_offsetof(GDesc, off_deg) == 16 && __builtin_offsetof(GDesc, U) == 32,
+              "GDesc word layout");
 
 // Mutable search state of one graph (one 64-byte line, so graphs do not
 // share lines).
@@ -64,6 +68,30 @@ struct Global {
     uint32_t timed_out;       // the deadline passed before the search finished
     uint64_t t0;              // wall clock at init (device clock ticks)
     uint32_t pad5[10];
+    // serving: device copy of the host's request tail / shutdown flag, and the
+    // time of the last host poll (polls are rate-limited to one per interval)
+    uint32_t host_tail;
+    uint32_t host_shutdown;
+    uint64_t last_poll;
+    uint32_t pad6[12];
+};
+
+// Serving mode: control words and completion records in pinned host memory.
+struct HostCtrl {
+    uint32_t tail;      // requests published so far (ring entries [0, tail))
+    uint32_t pad0[15];
+    uint32_t shutdown;  // 1 = exit once every admitted request is done
+    uint32_t pad1[15];
+};
+
+struct Completion {
+    uint32_t seq;       // GDesc.seq of the request, written last
+    uint32_t minimum;   // incumbent (MVC) / cover size found (PVC)
+    uint32_t k_found;
+    uint32_t flags;     // bit0 budget exhausted, bit1 cancelled
+    uint64_t nodes;
+    int32_t cover_size;
+    uint32_t pad;
 };
 
 constexpr unsigned long long kWaitOne = 1ull;
@@ -110,6 +138,17 @@ struct SolveArgs {
     uint32_t roots;         // number of root items handed out by next_root
     uint64_t limit_ticks;   // deadline = glob->t0 + limit_ticks (device wall clock)
     char *hbm_nodes;        // [workers][worker_bytes] node storage in HBM mode
+    // serving mode: roots come from a request ring published by a host
+    // thread in pinned host memory; the worker that claims a request copies
+    // its descriptor and graph into HBM (gdesc / gbuf, same word offsets)
+    uint32_t serving;
+    uint32_t ring_mask;        // ring entries - 1 (power of two)
+    const uint32_t *req_ring;  // pinned: [ring] graph slot of each request
+    const GDesc *host_desc;    // pinned: [slots] request descriptors
+    uint32_t *host_arena;      // pinned: request graphs in, cover bitmaps out
+    HostCtrl *host;            // pinned
+    Completion *comp;          // pinned: one record per slot
+    uint64_t poll_ticks;       // minimum interval between host polls
 };
 
 }  // namespace vcg

@@ -634,12 +634,105 @@ __device__ __forceinline__ void flush_nodes(const SolveArgs &a, const GDesc &gd,
     pending = 0;
 }
 
-__device__ __forceinline__ void finish_unit(const SolveArgs &a, uint32_t g)
+// One outstanding subtree of graph g is finished.  The last one marks the
+// graph done; in serving mode the whole wave then copies the cover bitmap to
+// the request's pinned host area and lane 0 writes the completion record
+// (fields first, the sequence number last, with a system-scope release in
+// between so the host only ever sees complete records).
+__device__ void finish_unit(const SolveArgs &a, uint32_t g, const GDesc &gd)
 {
+    GState *gs = &a.gstate[g];
+    uint32_t last = 0;
     if (lane_id() == 0) {
-        uint32_t old = fetch_add(&a.gstate[g].outstanding, 0xFFFFFFFFu);
-        if (old == 1u) st_rlx(&a.gstate[g].done, 1u);
+        uint32_t old = __hip_atomic_fetch_add(&gs->outstanding, 0xFFFFFFFFu, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        last = old == 1u;
+        if (last) st_rlx(&gs->done, 1u);
     }
+    if (!a.serving || !uni(last)) return;
+    fence_acq();
+    const uint32_t found = uni(ld_rlx(&gs->found));
+    const int32_t cs = (int32_t)uni((uint32_t)ld_rlx(&gs->cover_size));
+    const uint32_t best = uni((uint32_t)ld_rlx(&gs->best));
+    const bool has_cover = gd.instance == 0 ? (cs >= 0 && (uint32_t)cs == best) : found != 0u;
+    if (has_cover) {
+        const uint32_t words = ((gd.n + 63u) / 64u) * 2u;
+        for (uint32_t i = lane_id(); i < words; i += kWave)
+            __hip_atomic_store(&a.host_arena[gd.off_cover + i], a.covers[gd.off_cover + i], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    if (lane_id() == 0) {
+        Completion *c = &a.comp[g];
+        const int sys = __HIP_MEMORY_SCOPE_SYSTEM;
+        uint32_t minimum = gd.instance == 0 ? best : (found ? (uint32_t)cs : 0u);
+        uint32_t flags = (ld_rlx(&gs->budget_hit) ? 1u : 0u) | (ld_rlx(&a.glob->cancel) ? 2u : 0u) |
+                         (has_cover ? 4u : 0u);
+        __hip_atomic_store(&c->minimum, minimum, __ATOMIC_RELAXED, sys);
+        __hip_atomic_store(&c->k_found, found, __ATOMIC_RELAXED, sys);
+        __hip_atomic_store(&c->flags, flags, __ATOMIC_RELAXED, sys);
+        __hip_atomic_store(&c->nodes, ld_rlx(&gs->nodes), __ATOMIC_RELAXED, sys);
+        __hip_atomic_store(&c->cover_size, cs, __ATOMIC_RELAXED, sys);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    if (lane_id() == 0) __hip_atomic_store(&a.comp[g].seq, gd.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Serving: the worker that claimed request slot g brings it into HBM -- the
+// descriptor into gdesc[g], the graph words into the device arena at the
+// same offsets, and a fresh GState -- before anyone else can reach it (other
+// workers only see g through this worker's donations, which publish with
+// release/acquire on the worklist tickets).
+__device__ void load_request(const SolveArgs &a, uint32_t g)
+{
+    const uint32_t *hd = reinterpret_cast<const uint32_t *>(&a.host_desc[g]);
+    uint32_t *dd = reinterpret_cast<uint32_t *>(const_cast<GDesc *>(&a.gdesc[g]));
+    uint32_t word = 0;
+    if (lane_id() < 16u) {
+        word = __hip_atomic_load(&hd[lane_id()], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        dd[lane_id()] = word;
+    }
+    // GDesc words: 0 n, 2 off_src, 4 off_deg, 8 U
+    const uint32_t n = __builtin_amdgcn_readlane(word, 0);
+    const uint32_t lo = __builtin_amdgcn_readlane(word, 2);
+    const uint32_t hi = __builtin_amdgcn_readlane(word, 4) + n;
+    const uint32_t U = __builtin_amdgcn_readlane(word, 8);
+    uint32_t *dev = const_cast<uint32_t *>(a.gbuf);
+    for (uint32_t i = lo + lane_id(); i < hi; i += kWave) dev[i] = a.host_arena[i];
+    if (lane_id() == 0) {
+        GState s = {};
+        s.best = (int32_t)U;
+        s.outstanding = 1u;
+        s.cover_size = -1;
+        a.gstate[g] = s;
+    }
+    fence_rel();
+    fence_acq();
+}
+
+// Number of roots that may be handed out.  Batch: all of them.  Serving: the
+// host's request tail, polled at most once per poll interval by one worker
+// (system-scope acquire on pinned memory) and shared through a device copy.
+__device__ __forceinline__ uint32_t roots_available(const SolveArgs &a)
+{
+    if (!a.serving) return a.roots;
+    uint32_t t = 0;
+    if (lane_id() == 0) {
+        t = ld_rlx(&a.glob->host_tail);
+        uint64_t now = wall_clock64();
+        uint64_t last = ld_rlx(&a.glob->last_poll);
+        if (now - last > a.poll_ticks &&
+            __hip_atomic_compare_exchange_strong(&a.glob->last_poll, &last, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+            uint32_t ht = __hip_atomic_load(&a.host->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            uint32_t sd = __hip_atomic_load(&a.host->shutdown, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (ht > t) {
+                __hip_atomic_fetch_max(&a.glob->host_tail, ht, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                t = ht;
+            }
+            if (sd) st_rlx(&a.glob->host_shutdown, 1u);
+        }
+    }
+    return uni(t);
 }
 
 // Idle worker: take a queued item, else a new root, else wait; returns
@@ -665,17 +758,43 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
                 return r;
             }
         }
-        if (uni(ld_rlx(&a.glob->next_root)) < a.roots) {
+        const uint32_t avail = roots_available(a);
+        if (uni(ld_rlx(&a.glob->next_root)) < avail) {
             uint32_t idx = kNone;
             if (lane_id() == 0) {
                 fetch_add(&a.glob->counter, (unsigned long long)(0ull - kWaitOne));
-                idx = fetch_add(&a.glob->next_root, 1u);
-                if (idx >= a.roots) fetch_add(&a.glob->counter, kWaitOne);
+                if (!a.serving) {
+                    // batch: indices past the fixed root count are simply invalid
+                    idx = fetch_add(&a.glob->next_root, 1u);
+                } else {
+                    // serving: never move the counter past the published tail,
+                    // or a request published later would be skipped
+                    uint32_t cur = ld_rlx(&a.glob->next_root);
+                    while (cur < avail) {
+                        if (__hip_atomic_compare_exchange_strong(&a.glob->next_root, &cur, cur + 1u, __ATOMIC_RELAXED,
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                            idx = cur;
+                            break;
+                        }
+                    }
+                }
+                if (idx >= avail) fetch_add(&a.glob->counter, kWaitOne);
             }
             idx = uni(idx);
-            if (idx < a.roots) {
+            if (idx < avail) {
                 r.kind = 2;
                 r.root = idx;
+                if (a.serving) {
+                    // the host wrote the ring entry, descriptor and graph to
+                    // pinned memory before publishing the tail
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+                    uint32_t slot = 0;
+                    if (lane_id() == 0)
+                        slot = __hip_atomic_load(&a.req_ring[idx & a.ring_mask], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_SYSTEM);
+                    r.root = uni(slot);
+                    load_request(a, r.root);
+                }
                 return r;
             }
         }
@@ -683,7 +802,10 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
         if (lane_id() == 0) c = ld_rlx(&a.glob->counter);
         uint32_t lo = uni((uint32_t)c), hi = uni((uint32_t)(c >> 32));
         uint32_t waiting = lo & 0xFFFFu, reg = lo >> 16;
-        if (waiting == reg && hi == 0u && uni(ld_rlx(&a.glob->next_root)) >= a.roots) {
+        const bool no_more_roots = a.serving ? (uni(ld_rlx(&a.glob->host_shutdown)) != 0u &&
+                                                uni(ld_rlx(&a.glob->next_root)) >= avail)
+                                             : uni(ld_rlx(&a.glob->next_root)) >= a.roots;
+        if (waiting == reg && hi == 0u && no_more_roots) {
             if (lane_id() == 0) st_rlx(&a.glob->done_flag, 1u);
             return r;
         }
@@ -748,8 +870,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             if (top >= 0) {
                 // pop (popStack, src/stack.cuh:10-20), then build the stored
                 // right child
-                copy_in<HBM>(w.deg, stk + (size_t)top * slot_bytes, slot_bytes);
-                uint2 m = stk_meta[top];
+                const uint32_t ts = (uint32_t)top % a.depth;
+                copy_in<HBM>(w.deg, stk + (size_t)ts * slot_bytes, slot_bytes);
+                uint2 m = stk_meta[ts];
                 d = uni(m.x);
                 uint32_t v = uni(m.y);
                 --top;
@@ -759,7 +882,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 if (g != kNone) {
                     total_nodes += pending;
                     flush_nodes(a, gd, g, pending);
-                    finish_unit(a, g);
+                    finish_unit(a, g, gd);
                     g = kNone;
                 }
                 if (lane_id() == 0 && ld_rlx(&a.glob->cancel)) st_rlx(&a.glob->done_flag, 1u);
@@ -857,14 +980,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         }
         // Branch: the right child (N(v) into the cover) is stored as its
         // parent state plus v and built when it is popped.
-        if (top + 1 >= (int)a.depth) {
-            if (lane_id() == 0) st_rlx(&a.glob->error, kErrStackOverflow);
-            have = false;
-            continue;
+        // The local stack is a ring of `depth` slots holding entries
+        // bot..top.  With depth = min(n, max(U, k+1)) + 1 it never fills
+        // (an entry pushed at cover size d sits at index <= d); a smaller
+        // configured depth hands the oldest entry to the worklist instead.
+        if (top - bot + 1 >= (int)a.depth) {
+            bool moved = false;
+            if (a.version == 1) {
+                for (uint32_t spins = 0;; ++spins) {
+                    const uint32_t bs = (uint32_t)bot % a.depth;
+                    uint2 m = stk_meta[bs];
+                    if (try_enqueue(a, stk + (size_t)bs * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
+                        ++bot;
+                        ++donations;
+                        moved = true;
+                        break;
+                    }
+                    uint32_t late = 0;
+                    if (lane_id() == 0) late = past_deadline(a);
+                    if (uni(late)) break;
+                    backoff(spins < 4 ? spins : 4);
+                }
+            }
+            if (!moved) {
+                if (lane_id() == 0) st_rlx(&a.glob->error, kErrStackOverflow);
+                have = false;
+                continue;
+            }
         }
         ++top;
-        copy_out<HBM>(stk + (size_t)top * slot_bytes, w.deg, slot_bytes);
-        if (lane_id() == 0) stk_meta[top] = make_uint2(d, v);
+        const uint32_t ts = (uint32_t)top % a.depth;
+        copy_out<HBM>(stk + (size_t)ts * slot_bytes, w.deg, slot_bytes);
+        if (lane_id() == 0) stk_meta[ts] = make_uint2(d, v);
         if (a.version == 1) {
             // Donate while workers wait and the worklist holds fewer items
             // than there are waiting workers (and fewer than the -t
@@ -875,8 +1022,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             int waiting = (int)(uni((uint32_t)c) & 0xFFFFu);
             int enq = (int)uni((uint32_t)(c >> 32));
             if (waiting > 0 && enq < waiting && enq < (int)a.qthreshold && top >= bot) {
-                uint2 m = stk_meta[bot];
-                if (try_enqueue(a, stk + (size_t)bot * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
+                const uint32_t bs = (uint32_t)bot % a.depth;
+                uint2 m = stk_meta[bs];
+                if (try_enqueue(a, stk + (size_t)bs * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
                     ++bot;
                     ++donations;
                 }
@@ -1030,7 +1178,9 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes)
     pl.worker_bytes = round_up(pl.npad * deg_bytes + 4u * pl.lcap + 4u * pl.mwords, 64u);
 }
 
-vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *p, Plan &pl)
+// Launch shape shared by the batch solve and the server: node storage,
+// workgroup, grid from the occupancy query, worklist, and the memory plan.
+vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_params *p, Plan &pl)
 {
     const uint32_t version = p->version;
     if (version != VC_HYBRID && version != VC_STACK_ONLY) return VC_EINVAL;
@@ -1038,7 +1188,7 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
     uint32_t block_dim = p->block_dim ? p->block_dim : 256u;
     if (block_dim % vcg::kWave != 0 || block_dim > 256u) return VC_EINVAL;
     if (version == VC_STACK_ONLY && p->start_depth > 20u) return VC_EINVAL;
-    const uint32_t nmax = std::max<uint32_t>(set->nmax, 1u);
+    const uint32_t nmax = std::max<uint32_t>(nmax_in, 1u);
     const size_t lds_limit = ctx->lds_per_cu ? ctx->lds_per_cu : 160u * 1024u;
 
     // Degree arrays in LDS when at least one worker fits per CU (int16 while
@@ -1067,16 +1217,7 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
     pl.blocks_per_cu = (uint32_t)bpc;
     pl.num_blocks = p->num_blocks ? p->num_blocks : (uint32_t)bpc * (uint32_t)ctx->cu_count;
 
-    // Stack depth: the stack never holds more entries than the cover size of
-    // the current node, which stays below max(U, k+1) (prune at d >= U for
-    // MVC, d > k for PVC), and never more than n.
-    uint32_t need = 1;
-    for (int i = 0; i < set->count; ++i) {
-        uint32_t lim = jobs[i].instance == VC_MVC ? jobs[i].upper_bound : std::max(jobs[i].upper_bound, jobs[i].k + 1u);
-        lim = std::min(lim, set->h_desc[i].n);
-        need = std::max(need, lim);
-    }
-    pl.depth = need + 1u;
+    pl.depth = std::max<uint32_t>(depth, 1u);
 
     // Worklist: size and donation threshold (-q, -t).
     pl.workers = pl.num_blocks * pl.wpb;
@@ -1119,8 +1260,68 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
     if (pl.total > budget) return VC_ENOMEM;
 
     pl.units = version == VC_HYBRID ? 1u : (1u << p->start_depth);
+    pl.roots = 0;
+    return VC_OK;
+}
+
+vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *p, Plan &pl)
+{
+    // Stack depth: the stack never holds more entries than the cover size of
+    // the current node, which stays below max(U, k+1) (prune at d >= U for
+    // MVC, d > k for PVC), and never more than n.
+    uint32_t need = 1;
+    for (int i = 0; i < set->count; ++i) {
+        uint32_t lim = jobs[i].instance == VC_MVC ? jobs[i].upper_bound : std::max(jobs[i].upper_bound, jobs[i].k + 1u);
+        lim = std::min(lim, set->h_desc[i].n);
+        need = std::max(need, lim);
+    }
+    vc_status st = plan_launch(ctx, set->nmax, need + 1u, p, pl);
+    if (st != VC_OK) return st;
     pl.roots = (uint32_t)set->count * pl.units;
     return VC_OK;
+}
+
+// Fills the launch arguments that depend only on the plan and the scratch base.
+void fill_args(SolveArgs &a, const Plan &pl, char *base, uint32_t version, uint32_t start_depth, uint64_t limit_ticks)
+{
+    a.glob = reinterpret_cast<Global *>(base + pl.off_glob);
+    a.stk = base + pl.off_stk;
+    a.stk_meta = reinterpret_cast<uint2 *>(base + pl.off_meta);
+    a.q_deg = base + pl.off_qdeg;
+    a.q_meta = reinterpret_cast<uint4 *>(base + pl.off_qmeta);
+    a.tickets = reinterpret_cast<uint32_t *>(base + pl.off_tickets);
+    a.wstats = reinterpret_cast<vcg::WorkerStats *>(base + pl.off_wnodes);
+    a.npad = pl.npad;
+    a.lcap = pl.lcap;
+    a.mwords = pl.mwords;
+    a.worker_bytes = pl.worker_bytes;
+    a.depth = pl.depth;
+    a.qsize = pl.qsize;
+    a.qthreshold = pl.qthreshold;
+    a.workers = pl.workers;
+    a.version = version;
+    a.start_depth = version == VC_STACK_ONLY ? start_depth : 0u;
+    a.roots = pl.roots;
+    a.limit_ticks = limit_ticks;
+    a.hbm_nodes = pl.variant == kHbm32 ? base + pl.off_hbm : nullptr;
+}
+
+bool launch_solve(const Plan &pl, hipStream_t stream, const SolveArgs &a)
+{
+    switch (pl.variant) {
+    case kLds16:
+        hipLaunchKernelGGL((vcg::solve_kernel<int16_t, false>), dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem,
+                           stream, a);
+        break;
+    case kLds32:
+        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, false>), dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem,
+                           stream, a);
+        break;
+    default:
+        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, true>), dim3(pl.num_blocks), dim3(pl.block_dim), 0, stream, a);
+        break;
+    }
+    return hip_ok(hipGetLastError());
 }
 
 }  // namespace
@@ -1138,6 +1339,7 @@ const char *vc_status_string(vc_status s)
     case VC_ECANCELLED: return "cancelled";
     case VC_ENOTSUP: return "not supported";
     case VC_EBADGRAPH: return "invalid graph (duplicate edge, self-loop, unsorted or out-of-range adjacency)";
+    case VC_EAGAIN: return "request not finished yet";
     }
     return "unknown status";
 }
@@ -1320,50 +1522,17 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     a.gdesc = set->d_desc;
     a.gstate = set->d_state;
     a.covers = set->d_covers;
-    a.glob = reinterpret_cast<Global *>(base + pl.off_glob);
-    a.stk = base + pl.off_stk;
-    a.stk_meta = reinterpret_cast<uint2 *>(base + pl.off_meta);
-    a.q_deg = base + pl.off_qdeg;
-    a.q_meta = reinterpret_cast<uint4 *>(base + pl.off_qmeta);
-    a.tickets = reinterpret_cast<uint32_t *>(base + pl.off_tickets);
-    a.wstats = reinterpret_cast<vcg::WorkerStats *>(base + pl.off_wnodes);
     a.num_graphs = (uint32_t)set->count;
-    a.npad = pl.npad;
-    a.lcap = pl.lcap;
-    a.mwords = pl.mwords;
-    a.worker_bytes = pl.worker_bytes;
-    a.depth = pl.depth;
-    a.qsize = pl.qsize;
-    a.qthreshold = pl.qthreshold;
-    a.workers = pl.workers;
-    a.version = params->version;
-    a.start_depth = params->version == VC_STACK_ONLY ? params->start_depth : 0u;
-    a.roots = pl.roots;
     {
         // deadline on the device wall clock (every wait loop checks it)
         uint64_t ms = params->time_limit_ms ? params->time_limit_ms : 3600ull * 1000ull;
-        a.limit_ticks = ms * (uint64_t)ctx->wall_khz;
+        fill_args(a, pl, base, params->version, params->start_depth, ms * (uint64_t)ctx->wall_khz);
     }
-    a.hbm_nodes = pl.variant == kHbm32 ? base + pl.off_hbm : nullptr;
 
     hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, ctx->stream, a, pl.units);
     if (!hip_ok(hipGetLastError())) return VC_EDEVICE;
     hipEventRecord(ctx->ev0, ctx->stream);
-    switch (pl.variant) {
-    case kLds16:
-        hipLaunchKernelGGL((vcg::solve_kernel<int16_t, false>), dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem,
-                           ctx->stream, a);
-        break;
-    case kLds32:
-        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, false>), dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem,
-                           ctx->stream, a);
-        break;
-    default:
-        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, true>), dim3(pl.num_blocks), dim3(pl.block_dim), 0,
-                           ctx->stream, a);
-        break;
-    }
-    if (!hip_ok(hipGetLastError())) return VC_EDEVICE;
+    if (!launch_solve(pl, ctx->stream, a)) return VC_EDEVICE;
     hipEventRecord(ctx->ev1, ctx->stream);
 
     std::vector<GState> hs(set->count);
@@ -1437,6 +1606,487 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         info->idle_fraction = span > 0 ? (float)std::min(1.0, (double)idle / span) : 0.f;
     }
     return overall;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// serving: continuous batching over one persistent launch
+// ===========================================================================
+//
+// The host never calls the HIP runtime while the server kernel runs (small
+// copies may be executed as blit kernels, which could not be scheduled next
+// to a persistent kernel that holds every CU).  Requests are written into
+// pinned host memory -- graph words into the host arena, descriptor into the
+// host descriptor table, slot id into the request ring -- and published by a
+// release store of the ring tail.  The worker that claims a request copies
+// it into HBM; completions and cover bitmaps come back the same way.
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+// First-fit allocator over 64-byte units of the arena.
+class ArenaAlloc {
+  public:
+    void init(uint64_t units)
+    {
+        free_.clear();
+        if (units) free_[0] = units;
+    }
+    bool alloc(uint64_t units, uint64_t &off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < units) continue;
+            off = it->first;
+            uint64_t rest = it->second - units;
+            free_.erase(it);
+            if (rest) free_[off + units] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t units)
+    {
+        auto it = free_.emplace(off, units).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+  private:
+    std::map<uint64_t, uint64_t> free_;
+};
+
+struct ServerRequest {
+    uint64_t ticket = 0;
+    vc_job job{};
+    uint32_t n = 0, m = 0;
+    std::vector<uint32_t> words;  // srcPtr[n+1] | dst[2m] | degree[n] | pad | cover bitmap
+    uint32_t off_dst = 0, off_deg = 0, off_cover = 0, cover_words = 0;
+};
+
+struct ServerSlot {
+    bool busy = false;
+    uint64_t ticket = 0;
+    uint32_t seq = 0;
+    uint64_t arena_off = 0, arena_units = 0;
+    uint32_t n = 0, cover_word = 0;
+    vc_job job{};
+};
+
+struct ServerDone {
+    vc_result r{};
+    std::vector<uint8_t> cover;
+    uint64_t done_ns = 0;
+};
+
+uint64_t now_ns()
+{
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+constexpr uint64_t kUnit = 64;
+
+}  // namespace
+
+struct vc_server {
+    vc_ctx *ctx = nullptr;
+    vc_server_config cfg{};
+    Plan pl{};
+    SolveArgs a{};
+    hipStream_t kstream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *scratch = nullptr;
+    GDesc *d_desc = nullptr;
+    GState *d_state = nullptr;
+    char *d_arena = nullptr;
+    // pinned host memory shared with the kernel
+    vcg::HostCtrl *host = nullptr;
+    vcg::Completion *comp = nullptr;
+    GDesc *h_desc = nullptr;
+    uint32_t *h_ring = nullptr;
+    uint32_t *h_arena = nullptr;
+    uint32_t ring_size = 0;
+
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::unique_ptr<ServerRequest>> queue;  // submitted, not admitted
+    std::vector<ServerSlot> slots;
+    std::vector<uint32_t> free_slots;
+    std::vector<uint32_t> busy_list;
+    ArenaAlloc arena;
+    std::unordered_map<uint64_t, ServerDone> done;
+    uint64_t next_ticket = 1;
+    uint32_t tail = 0;
+    uint32_t seq = 0;
+    bool stopping = false;
+    bool dead = false;  // the kernel exited (deadline) or failed
+    std::thread th;
+};
+
+namespace {
+
+// Admission: move queued requests into free slots while the arena has room,
+// write them to pinned memory, publish the new tail.  Returns the number
+// admitted.
+uint32_t server_admit(vc_server *s)
+{
+    uint32_t admitted = 0;
+    std::lock_guard<std::mutex> lk(s->mu);
+    while (!s->queue.empty() && !s->free_slots.empty()) {
+        ServerRequest &rq = *s->queue.front();
+        const uint64_t units = (rq.words.size() * sizeof(uint32_t) + kUnit - 1) / kUnit;
+        uint64_t off = 0;
+        if (!s->arena.alloc(units, off)) break;  // admission by memory footprint
+        const uint32_t slot = s->free_slots.back();
+        s->free_slots.pop_back();
+        ServerSlot &sl = s->slots[slot];
+        sl.busy = true;
+        sl.ticket = rq.ticket;
+        sl.seq = ++s->seq;
+        if (sl.seq == 0) sl.seq = ++s->seq;
+        sl.arena_off = off;
+        sl.arena_units = units;
+        sl.n = rq.n;
+        sl.job = rq.job;
+        const uint32_t base = (uint32_t)(off * (kUnit / 4));
+        sl.cover_word = base + rq.off_cover;
+        std::memcpy(s->h_arena + base, rq.words.data(), rq.words.size() * sizeof(uint32_t));
+        GDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.n = rq.n;
+        d.m = rq.m;
+        d.off_src = base;
+        d.off_dst = base + rq.off_dst;
+        d.off_deg = base + rq.off_deg;
+        d.off_cover = sl.cover_word;
+        d.instance = rq.job.instance;
+        d.k = rq.job.k;
+        d.U = rq.job.upper_bound;
+        d.seq = sl.seq;
+        d.max_nodes = rq.job.max_nodes;
+        s->h_desc[slot] = d;
+        s->h_ring[s->tail & (s->ring_size - 1u)] = slot;
+        ++s->tail;
+        s->busy_list.push_back(slot);
+        s->queue.pop_front();
+        ++admitted;
+    }
+    // publish: every write above precedes the tail the kernel acquires
+    if (admitted) __atomic_store_n(&s->host->tail, s->tail, __ATOMIC_RELEASE);
+    return admitted;
+}
+
+// Collects finished requests.  Returns how many finished.
+uint32_t server_collect(vc_server *s)
+{
+    uint32_t finished = 0;
+    const uint64_t t_done = now_ns();
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (size_t i = 0; i < s->busy_list.size();) {
+        const uint32_t slot = s->busy_list[i];
+        ServerSlot &sl = s->slots[slot];
+        if (__atomic_load_n(&s->comp[slot].seq, __ATOMIC_ACQUIRE) != sl.seq) {
+            ++i;
+            continue;
+        }
+        const vcg::Completion &c = s->comp[slot];
+        ServerDone dn;
+        vc_result &r = dn.r;
+        std::memset(&r, 0, sizeof(r));
+        r.tree_nodes = c.nodes;
+        const bool budget = (c.flags & 1u) != 0u, cancelled = (c.flags & 2u) != 0u;
+        r.minimum = c.minimum;
+        if (sl.job.instance == VC_MVC) {
+            r.solved = !budget && !cancelled;
+        } else {
+            r.k_found = c.k_found;
+            r.solved = c.k_found || (!budget && !cancelled);
+        }
+        r.cover_from_gpu = (c.flags & 4u) != 0u;
+        r.status = r.solved ? VC_OK : VC_ETIMEOUT;
+        if (r.cover_from_gpu) {
+            dn.cover.resize(sl.n);
+            const uint32_t *bm = s->h_arena + sl.cover_word;
+            for (uint32_t v = 0; v < sl.n; ++v) dn.cover[v] = (bm[v >> 5] >> (v & 31u)) & 1u;
+        }
+        dn.done_ns = t_done;
+        s->done[sl.ticket] = std::move(dn);
+        s->arena.release(sl.arena_off, sl.arena_units);
+        sl.busy = false;
+        s->free_slots.push_back(slot);
+        s->busy_list[i] = s->busy_list.back();
+        s->busy_list.pop_back();
+        ++finished;
+    }
+    if (finished) s->cv.notify_all();
+    return finished;
+}
+
+void server_loop(vc_server *s)
+{
+    hipSetDevice(s->ctx->device);
+    uint32_t idle_rounds = 0;
+    for (;;) {
+        uint32_t work = server_admit(s) + server_collect(s);
+        // a cheap liveness check every few idle rounds (a stream query is not
+        // a copy and needs no CU)
+        bool alive = true;
+        if (!work && ++idle_rounds % 64 == 0) alive = hipStreamQuery(s->kstream) == hipErrorNotReady;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (!alive) s->dead = true;
+            if (s->dead || (s->stopping && s->queue.empty() && s->busy_list.empty())) break;
+        }
+        if (!work) std::this_thread::sleep_for(std::chrono::microseconds(10));
+    }
+    // let the kernel drain and exit
+    __atomic_store_n(&s->host->shutdown, 1u, __ATOMIC_RELEASE);
+    hipStreamSynchronize(s->kstream);
+    server_collect(s);
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto fail = [&](uint64_t ticket) {
+        ServerDone dn;
+        std::memset(&dn.r, 0, sizeof(dn.r));
+        dn.r.status = VC_ETIMEOUT;
+        dn.done_ns = now_ns();
+        s->done[ticket] = std::move(dn);
+    };
+    for (uint32_t slot : s->busy_list) fail(s->slots[slot].ticket);
+    s->busy_list.clear();
+    for (auto &rq : s->queue) fail(rq->ticket);
+    s->queue.clear();
+    s->dead = true;
+    s->cv.notify_all();
+}
+
+void server_free(vc_server *s)
+{
+    if (s->scratch) hipFree(s->scratch);
+    if (s->d_desc) hipFree(s->d_desc);
+    if (s->d_state) hipFree(s->d_state);
+    if (s->d_arena) hipFree(s->d_arena);
+    if (s->host) hipHostFree(s->host);
+    if (s->comp) hipHostFree(s->comp);
+    if (s->h_desc) hipHostFree(s->h_desc);
+    if (s->h_ring) hipHostFree(s->h_ring);
+    if (s->h_arena) hipHostFree(s->h_arena);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->kstream) hipStreamDestroy(s->kstream);
+    delete s;
+}
+
+template <class T>
+bool pinned_alloc(T **p, size_t bytes)
+{
+    return hip_ok(hipHostMalloc(reinterpret_cast<void **>(p), bytes, hipHostMallocCoherent | hipHostMallocMapped));
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t vc_now_ns(void) { return now_ns(); }
+
+vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **out)
+{
+    if (!ctx || !cfg || !out || cfg->max_vertices == 0) return VC_EINVAL;
+    *out = nullptr;
+    hipSetDevice(ctx->device);
+    vc_server *s = new vc_server();
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    const uint32_t slots = cfg->slots ? cfg->slots : 4096u;
+    const uint64_t arena_bytes = cfg->arena_bytes ? cfg->arena_bytes : (1ull << 30);
+    if (arena_bytes / 4 >= 0xFFFFFFF0ull) {
+        delete s;
+        return VC_EINVAL;
+    }
+    vc_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.version = VC_HYBRID;
+    p.block_dim = cfg->block_dim;
+    p.num_blocks = cfg->num_blocks;
+    p.use_global_memory = -1;
+    const uint32_t depth = cfg->stack_depth ? cfg->stack_depth : cfg->max_vertices + 1u;
+    vc_status st = plan_launch(ctx, cfg->max_vertices, depth, &p, s->pl);
+    if (st != VC_OK) {
+        delete s;
+        return st;
+    }
+    s->ring_size = next_pow2(std::max<uint32_t>(2u * slots, 1024u));
+    bool ok = hip_ok(hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking)) &&
+              hip_ok(hipEventCreate(&s->ev0)) && hip_ok(hipEventCreate(&s->ev1)) &&
+              hip_ok(hipMalloc(&s->scratch, s->pl.total)) && hip_ok(hipMalloc(&s->d_desc, sizeof(GDesc) * slots)) &&
+              hip_ok(hipMalloc(&s->d_state, sizeof(GState) * slots)) && hip_ok(hipMalloc(&s->d_arena, arena_bytes)) &&
+              pinned_alloc(&s->host, sizeof(vcg::HostCtrl)) && pinned_alloc(&s->comp, sizeof(vcg::Completion) * slots) &&
+              pinned_alloc(&s->h_desc, sizeof(GDesc) * slots) && pinned_alloc(&s->h_ring, sizeof(uint32_t) * s->ring_size) &&
+              pinned_alloc(&s->h_arena, arena_bytes);
+    if (!ok) {
+        server_free(s);
+        return VC_ENOMEM;
+    }
+    std::memset(s->host, 0, sizeof(vcg::HostCtrl));
+    std::memset(s->comp, 0, sizeof(vcg::Completion) * slots);
+    std::memset(s->h_desc, 0, sizeof(GDesc) * slots);
+    s->slots.resize(slots);
+    for (uint32_t i = 0; i < slots; ++i) s->free_slots.push_back(slots - 1 - i);
+    s->arena.init(arena_bytes / kUnit);
+
+    SolveArgs &a = s->a;
+    std::memset(&a, 0, sizeof(a));
+    a.gbuf = reinterpret_cast<const uint32_t *>(s->d_arena);
+    a.covers = reinterpret_cast<uint32_t *>(s->d_arena);
+    a.gdesc = s->d_desc;
+    a.gstate = s->d_state;
+    a.num_graphs = 0;  // slot state is set by the worker that claims a request
+    const uint64_t ms = cfg->time_limit_ms ? cfg->time_limit_ms : 3600ull * 1000ull;
+    fill_args(a, s->pl, static_cast<char *>(s->scratch), VC_HYBRID, 0u, ms * (uint64_t)ctx->wall_khz);
+    a.serving = 1;
+    a.ring_mask = s->ring_size - 1u;
+    a.req_ring = s->h_ring;
+    a.host_desc = s->h_desc;
+    a.host_arena = s->h_arena;
+    a.host = s->host;
+    a.comp = s->comp;
+    a.poll_ticks = std::max<uint64_t>(1, (uint64_t)ctx->wall_khz * 2u / 1000u);  // about 2 microseconds
+    hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, s->kstream, a, 1u);
+    ok = hip_ok(hipGetLastError()) && hip_ok(hipEventRecord(s->ev0, s->kstream)) &&
+         launch_solve(s->pl, s->kstream, a) && hip_ok(hipEventRecord(s->ev1, s->kstream));
+    if (!ok) {
+        hipStreamSynchronize(s->kstream);
+        server_free(s);
+        return VC_EDEVICE;
+    }
+    s->th = std::thread(server_loop, s);
+    *out = s;
+    return VC_OK;
+}
+
+vc_status vc_server_submit(vc_server *s, const vc_csr *g, const vc_job *job, uint64_t *ticket)
+{
+    if (!s || !g || !job || !ticket) return VC_EINVAL;
+    if (g->vertexNum > s->cfg.max_vertices || g->vertexNum == 0) return VC_EINVAL;
+    if (job->instance != VC_MVC && job->instance != VC_PVC) return VC_EINVAL;
+    if (job->upper_bound > g->vertexNum) return VC_EINVAL;
+    if (!csr_is_valid(*g)) return VC_EBADGRAPH;
+    auto rq = std::make_unique<ServerRequest>();
+    rq->job = *job;
+    const uint32_t n = g->vertexNum, m = g->edgeNum;
+    rq->n = n;
+    rq->m = m;
+    rq->off_dst = n + 1u;
+    rq->off_deg = rq->off_dst + 2u * m;
+    rq->off_cover = round_up(rq->off_deg + n, 16u);
+    rq->cover_words = round_up((n + 31u) / 32u, 2u);
+    rq->words.assign((size_t)rq->off_cover + rq->cover_words, 0u);
+    std::memcpy(rq->words.data(), g->srcPtr, sizeof(uint32_t) * (n + 1u));
+    if (m) std::memcpy(rq->words.data() + rq->off_dst, g->dst, sizeof(uint32_t) * 2ull * m);
+    std::memcpy(rq->words.data() + rq->off_deg, g->degree, sizeof(int32_t) * n);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->dead || s->stopping) return VC_ECANCELLED;
+    rq->ticket = s->next_ticket++;
+    *ticket = rq->ticket;
+    s->queue.push_back(std::move(rq));
+    return VC_OK;
+}
+
+static vc_status take_done(vc_server *s, uint64_t ticket, vc_result *result, uint8_t *cover, uint64_t *done_ns)
+{
+    auto it = s->done.find(ticket);
+    if (it == s->done.end()) return VC_EAGAIN;
+    if (result) *result = it->second.r;
+    if (cover && !it->second.cover.empty()) std::memcpy(cover, it->second.cover.data(), it->second.cover.size());
+    if (done_ns) *done_ns = it->second.done_ns;
+    s->done.erase(it);
+    return VC_OK;
+}
+
+vc_status vc_server_poll(vc_server *s, uint64_t ticket, vc_result *result, uint8_t *cover, uint64_t *done_ns)
+{
+    if (!s) return VC_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return take_done(s, ticket, result, cover, done_ns);
+}
+
+vc_status vc_server_wait(vc_server *s, uint64_t ticket, uint32_t timeout_ms, vc_result *result, uint8_t *cover,
+                         uint64_t *done_ns)
+{
+    if (!s) return VC_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeout_ms);
+    for (;;) {
+        vc_status st = take_done(s, ticket, result, cover, done_ns);
+        if (st != VC_EAGAIN) return st;
+        if (s->cv.wait_until(lk, deadline) == std::cv_status::timeout)
+            return take_done(s, ticket, result, cover, done_ns);
+    }
+}
+
+vc_status vc_server_stop(vc_server *s, vc_launch_info *info)
+{
+    if (!s) return VC_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    if (s->th.joinable()) s->th.join();
+    vc_status st = hip_ok(hipStreamSynchronize(s->kstream)) ? VC_OK : VC_EDEVICE;
+    std::vector<vcg::WorkerStats> wn(s->pl.workers);
+    Global hg;
+    std::memset(&hg, 0, sizeof(hg));
+    bool got = hip_ok(hipMemcpy(wn.data(), s->a.wstats, sizeof(vcg::WorkerStats) * s->pl.workers,
+                                hipMemcpyDeviceToHost)) &&
+               hip_ok(hipMemcpy(&hg, s->a.glob, sizeof(Global), hipMemcpyDeviceToHost));
+    if (!got) st = VC_EDEVICE;
+    if (hg.error) st = VC_EDEVICE;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        uint64_t idle = 0;
+        for (const auto &x : wn) {
+            info->tree_nodes += x.nodes;
+            idle += x.idle_ticks;
+            info->donations += x.donations;
+            info->steals += x.steals;
+        }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, s->ev0, s->ev1);
+        info->kernel_ms = ms;
+        double span = (double)ms * 1e-3 * (double)s->ctx->wall_khz * 1e3 * (double)s->pl.workers;
+        info->idle_fraction = span > 0 ? (float)std::min(1.0, (double)idle / span) : 0.f;
+        info->num_blocks = s->pl.num_blocks;
+        info->block_dim = s->pl.block_dim;
+        info->workers = s->pl.workers;
+        info->blocks_per_cu = s->pl.blocks_per_cu;
+        info->use_global_memory = s->pl.variant == kHbm32;
+        info->list_size = s->pl.qsize;
+        info->list_threshold = s->pl.qthreshold;
+        info->stack_depth = s->pl.depth;
+        info->cu_count = (uint32_t)s->ctx->cu_count;
+    }
+    server_free(s);
+    return st;
 }
 
 }  // extern "C"

@@ -63,12 +63,20 @@ class _Info(C.Structure):
                 ("steals", C.c_uint64), ("idle_fraction", C.c_float), ("reserved", C.c_uint32)]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("max_vertices", C.c_uint32), ("slots", C.c_uint32), ("arena_bytes", C.c_uint64),
+                ("num_blocks", C.c_uint32), ("block_dim", C.c_uint32), ("stack_depth", C.c_uint32),
+                ("time_limit_ms", C.c_uint32)]
+
+
 # Every symbol include/vcgpu.h declares.
 EXPORTS = [
     "vc_ctx_create", "vc_ctx_destroy", "vc_graphs_upload", "vc_graphs_free", "vc_solve", "vc_occupancy",
     "vc_load_graph_file", "vc_csr_from_edges", "vc_csr_free", "vc_check_graph", "vc_approx_remove_max",
-    "vc_approx_remove_edge", "vc_status_string",
+    "vc_approx_remove_edge", "vc_status_string", "vc_server_start", "vc_server_submit", "vc_server_poll",
+    "vc_server_wait", "vc_server_stop", "vc_now_ns",
 ]
+VC_EAGAIN = 8
 
 
 class VCError(RuntimeError):
@@ -105,6 +113,14 @@ def lib():
         L.vc_approx_remove_edge.restype = C.c_uint32
         L.vc_status_string.argtypes = [C.c_int]
         L.vc_status_string.restype = C.c_char_p
+        L.vc_server_start.argtypes = [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]
+        L.vc_server_submit.argtypes = [C.c_void_p, C.POINTER(_Csr), C.POINTER(_Job), C.POINTER(C.c_uint64)]
+        L.vc_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(_Result), _u8p, C.POINTER(C.c_uint64)]
+        L.vc_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(_Result), _u8p,
+                                     C.POINTER(C.c_uint64)]
+        L.vc_server_stop.argtypes = [C.c_void_p, C.POINTER(_Info)]
+        L.vc_now_ns.argtypes = []
+        L.vc_now_ns.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -352,3 +368,59 @@ def solve_pvc(ctx: Context, g: CSR, k: int, params: Params | None = None, seed: 
     finally:
         gs.free()
     return r.k_found, r.cover, r, info
+
+
+def now_ns() -> int:
+    """The host monotonic clock the server stamps completions with."""
+    return int(lib().vc_now_ns())
+
+
+class Server:
+    """Continuous-batching server (vc_server_*): one persistent launch,
+    requests admitted as slots and arena memory free up."""
+
+    def __init__(self, ctx: Context, max_vertices: int, slots: int = 0, arena_bytes: int = 0, num_blocks: int = 0,
+                 block_dim: int = 0, stack_depth: int = 0, time_limit_ms: int = 0):
+        cfg = _ServerConfig(max_vertices, slots, arena_bytes, num_blocks, block_dim, stack_depth, time_limit_ms)
+        h = C.c_void_p()
+        _check(lib().vc_server_start(ctx.handle, C.byref(cfg), C.byref(h)), "vc_server_start")
+        self.handle = h
+        self._n = {}
+
+    def submit(self, g: CSR, job: Job) -> int:
+        c = g._c()
+        j = _Job(job.instance, job.k, job.upper_bound, 0, job.max_nodes)
+        t = C.c_uint64()
+        _check(lib().vc_server_submit(self.handle, C.byref(c), C.byref(j), C.byref(t)), "vc_server_submit")
+        self._n[t.value] = g.n
+        return t.value
+
+    def _take(self, fn, ticket, *extra, want_cover=True):
+        n = self._n.get(ticket, 0)
+        buf = np.zeros(max(n, 1), np.uint8)
+        r = _Result()
+        done = C.c_uint64()
+        st = fn(self.handle, ticket, *extra, C.byref(r), buf.ctypes.data_as(_u8p) if want_cover else None,
+                C.byref(done))
+        if st == VC_EAGAIN:
+            return None
+        _check(st, "vc_server_poll")
+        self._n.pop(ticket, None)
+        cover = buf[:n].astype(bool) if (want_cover and r.cover_from_gpu) else None
+        res = Result(int(r.minimum), int(r.k_found), bool(r.solved), int(r.tree_nodes), int(r.status), cover)
+        return res, int(done.value)
+
+    def poll(self, ticket: int, want_cover: bool = True):
+        """(Result, done_ns) or None while pending."""
+        return self._take(lib().vc_server_poll, ticket, want_cover=want_cover)
+
+    def wait(self, ticket: int, timeout_ms: int = 60000, want_cover: bool = True):
+        """(Result, done_ns) or None on timeout."""
+        return self._take(lib().vc_server_wait, ticket, timeout_ms, want_cover=want_cover)
+
+    def stop(self) -> LaunchInfo:
+        info = _Info()
+        st = lib().vc_server_stop(self.handle, C.byref(info))
+        self.handle = None
+        _check(st, "vc_server_stop")
+        return LaunchInfo(*[getattr(info, f) for f, _ in _Info._fields_])

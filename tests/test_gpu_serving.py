@@ -1,0 +1,100 @@
+"""Serving (continuous batching) on the GPU: each request's answer is the same
+alone (its own one-graph launch) and in the mix (admitted into a running
+persistent kernel with other requests), and equal to the oracle's.
+
+Admission is forced to be continuous: fewer request slots than requests and a
+small graph arena, so requests are admitted as earlier ones finish."""
+import time
+
+import numpy as np
+import pytest
+
+import graphgen
+import oracle as O
+import vcgpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    cases = []
+    for s in range(12):
+        n = 40 + 7 * s
+        cases.append((n, graphgen.erdos_renyi(n, 0.08, 500 + s)))
+    for s in range(8):
+        n = 100 + 30 * s
+        cases.append((n, graphgen.barabasi_albert(n, 2 + s % 3, 600 + s)))
+    for s in range(4):
+        cases.append((120, graphgen.regular3_rewired(120, 10 + s)))
+    reqs = []
+    for i, (n, e) in enumerate(cases):
+        og = O.from_edges(n, e)
+        opt, _, _ = O.mvc(og)
+        g = vcgpu.csr_from_edges(n, e)
+        U, ucov = vcgpu.upper_bound(g, seed=9)
+        reqs.append((g, og, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), opt, ucov))
+        k = opt - 1 if i % 2 else opt + 1
+        reqs.append((g, og, vcgpu.Job(vcgpu.VC_PVC, k, U, 0), opt, ucov))
+    return reqs
+
+
+def _check(res, og, job, opt, ucov):
+    assert res.solved
+    if job.instance == vcgpu.VC_MVC:
+        assert res.minimum == opt
+        cover = res.cover if res.cover is not None else ucov
+        assert O.cover_is_valid(og, np.asarray(cover, np.uint8)) and int(np.sum(cover)) == opt
+    else:
+        assert res.k_found == int(job.k >= opt)
+        if res.k_found:
+            assert res.cover is not None
+            assert O.cover_is_valid(og, np.asarray(res.cover, np.uint8)) and int(np.sum(res.cover)) <= job.k
+
+
+def test_served_results_match_alone_and_oracle(ctx):
+    reqs = _requests()
+    # alone: one launch per request
+    alone = []
+    params = vcgpu.Params(time_limit_ms=60000)
+    for g, og, job, opt, ucov in reqs:
+        gs = ctx.upload(g)
+        (r,), _, st = ctx.solve(gs, job, params)
+        gs.free()
+        _check(r, og, job, opt, ucov)
+        alone.append(r)
+    # in the mix: 8 slots and a 1 MiB arena for 48 requests, submitted in two waves
+    srv = vcgpu.Server(ctx, max_vertices=512, slots=8, arena_bytes=1 << 20, time_limit_ms=60000)
+    try:
+        tickets = [srv.submit(g, job) for g, _, job, _, _ in reqs[:24]]
+        time.sleep(0.05)
+        tickets += [srv.submit(g, job) for g, _, job, _, _ in reqs[24:]]
+        for t, (g, og, job, opt, ucov), a in zip(tickets, reqs, alone):
+            out = srv.wait(t, 60000)
+            assert out is not None, "request did not finish"
+            r, done_ns = out
+            _check(r, og, job, opt, ucov)
+            if job.instance == vcgpu.VC_MVC:
+                assert r.minimum == a.minimum
+            else:
+                assert r.k_found == a.k_found
+    finally:
+        info = srv.stop()
+    assert info.tree_nodes > 0
+
+
+def test_server_rejects_oversized_and_reports_pending(ctx):
+    srv = vcgpu.Server(ctx, max_vertices=64, slots=4, time_limit_ms=30000)
+    try:
+        big = vcgpu.csr_from_edges(100, graphgen.erdos_renyi(100, 0.05, 1))
+        with pytest.raises(vcgpu.VCError):
+            srv.submit(big, vcgpu.Job(vcgpu.VC_MVC, 0, 100, 0))
+        n, e = 50, graphgen.erdos_renyi(50, 0.1, 2)
+        g = vcgpu.csr_from_edges(n, e)
+        U, _ = vcgpu.upper_bound(g)
+        t = srv.submit(g, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0))
+        out = srv.wait(t, 30000)
+        assert out is not None and out[0].solved
+        assert out[0].minimum == O.mvc(O.from_edges(n, e))[0]
+        assert srv.poll(t) is None  # a ticket is forgotten once its result was taken
+    finally:
+        srv.stop()
