@@ -93,12 +93,130 @@ def cpu_baseline_single(n, e, budget_s: float):
                       f"({'complete' if st['completed'] else 'capped'}) in {secs:.1f} s on 1 core"}
 
 
+def served(ctx, args, out):
+    """Served mixed stream: Poisson arrivals of MVC / PVC(k = opt +- 1)
+    requests, 95% graphs of the batch distribution and 5% Barabasi-Albert
+    n=20000 m=4, through two continuous-batching servers (small and large
+    graph classes) that share the GPU.  opt is precomputed offline by a batch
+    solve of the same graphs (the "alone" answers); graphs whose opt that
+    precompute cannot establish within its budget are not drawn, since the
+    PVC requests need opt."""
+    import threading
+    rng = np.random.Generator(np.random.PCG64(args.base_seed + 77))
+    n_req = args.requests
+    n_large = max(1, n_req // 20)
+    # candidate pool and offline opt
+    small_cands = [graphgen.batch_graph(i, args.base_seed)[:2] for i in range(min(3 * n_req, 4096))]
+    large_cands = [(20000, graphgen.barabasi_albert(20000, 4, 5000 + j)) for j in range(min(n_large, 8))]
+    pool = [vcgpu.csr_from_edges(n, e) for n, e in small_cands + large_cands]
+    ubs = [vcgpu.upper_bound(g, seed=args.base_seed) for g in pool]
+    t0 = time.perf_counter()
+    res_small, _, _ = ctx.solve(ctx.upload(pool[:len(small_cands)]),
+                                [vcgpu.Job(vcgpu.VC_MVC, 0, u, 1 << 18) for u, _ in ubs[:len(small_cands)]],
+                                vcgpu.Params(time_limit_ms=args.time_limit_ms), want_covers=False,
+                                raise_on_error=False)
+    res_large, _, _ = ctx.solve(ctx.upload(pool[len(small_cands):]),
+                                [vcgpu.Job(vcgpu.VC_MVC, 0, u, 1 << 18) for u, _ in ubs[len(small_cands):]],
+                                vcgpu.Params(time_limit_ms=args.time_limit_ms), want_covers=False,
+                                raise_on_error=False)
+    precompute_s = time.perf_counter() - t0
+    opts = [r.minimum if r.solved else None for r in list(res_small) + list(res_large)]
+    small_ok = [i for i in range(len(small_cands)) if opts[i] is not None]
+    large_ok = [len(small_cands) + j for j in range(len(large_cands)) if opts[len(small_cands) + j] is not None]
+    # the stream
+    reqs = []
+    for r in range(n_req):
+        big = large_ok and rng.random() < 0.05
+        gi = int(rng.choice(large_ok)) if big else int(small_ok[int(rng.integers(len(small_ok)))])
+        opt = opts[gi]
+        if rng.random() < 0.5:
+            job = vcgpu.Job(vcgpu.VC_MVC, 0, ubs[gi][0], 0)
+            want = ("mvc", opt)
+        else:
+            k = max(0, opt + (1 if rng.random() < 0.5 else -1))
+            job = vcgpu.Job(vcgpu.VC_PVC, k, ubs[gi][0], 0)
+            want = ("pvc", int(k >= opt))
+        reqs.append((gi, job, want))
+    gaps = rng.exponential(1.0 / args.rate, size=n_req)
+    # two servers: the small class takes most of each CU, the large class the rest
+    small_srv = vcgpu.Server(ctx, max_vertices=512, slots=4096, arena_bytes=256 << 20,
+                             num_blocks=args.small_blocks, time_limit_ms=args.time_limit_ms)
+    large_srv = vcgpu.Server(ctx, max_vertices=20000, slots=64, arena_bytes=256 << 20, block_dim=64,
+                             num_blocks=args.large_blocks, stack_depth=64, time_limit_ms=args.time_limit_ms)
+    arrivals = [0] * n_req
+    tickets = [None] * n_req
+    results = [None] * n_req
+
+    def collect():
+        for r in range(n_req):
+            while tickets[r] is None:
+                time.sleep(0.0002)
+            srv, t = tickets[r]
+            out = srv.wait(t, args.time_limit_ms, want_cover=False)
+            results[r] = out
+
+    th = threading.Thread(target=collect)
+    th.start()
+    start = vcgpu.now_ns()
+    due = start
+    for r, (gi, job, want) in enumerate(reqs):
+        due += int(gaps[r] * 1e9)
+        while True:
+            left = due - vcgpu.now_ns()
+            if left <= 0:
+                break
+            if left > 300_000:
+                time.sleep((left - 200_000) / 1e9)
+        srv = large_srv if pool[gi].n > 512 else small_srv
+        arrivals[r] = vcgpu.now_ns()
+        tickets[r] = (srv, srv.submit(pool[gi], job))
+    th.join()
+    info_s = small_srv.stop()
+    info_l = large_srv.stop()
+    lat, mism, unsolved, last = [], 0, 0, start
+    for r, (gi, job, want) in enumerate(reqs):
+        if results[r] is None:
+            unsolved += 1
+            continue
+        res, done_ns = results[r]
+        lat.append((done_ns - arrivals[r]) / 1e6)
+        last = max(last, done_ns)
+        if not res.solved:
+            unsolved += 1
+            continue
+        got = res.minimum if want[0] == "mvc" else res.k_found
+        mism += int(got != want[1])
+    wall = (last - start) / 1e9
+    lat = np.asarray(lat)
+    solved = n_req - unsolved
+    out.update({
+        "value": solved / wall, "unit": "graphs/s", "ms_per_step": wall * 1000.0,
+        "config": {"workload": f"served stream, Poisson {args.rate:g} req/s, {n_req} requests, 95% batch-distribution "
+                               f"graphs + 5% BA n=20000 m=4, 50/50 MVC / PVC k=opt+-1",
+                   "requests": n_req, "solved": solved, "mismatches_vs_alone": mism,
+                   "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+                   "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+                   "mean_ms": float(lat.mean()) if len(lat) else None,
+                   "large_requests": sum(1 for gi, _, _ in reqs if pool[gi].n > 512),
+                   "precompute_s": precompute_s, "small_workers": info_s.workers, "large_workers": info_l.workers,
+                   "tree_nodes": info_s.tree_nodes + info_l.tree_nodes,
+                   "idle_fraction_small": info_s.idle_fraction},
+        "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                     "traffic": None},
+    })
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batch", choices=["batch", "single_er", "single_reg3", "single_ba"])
+    ap.add_argument("--workload", default="batch",
+                    choices=["batch", "single_er", "single_reg3", "single_ba", "served"])
+    ap.add_argument("--requests", type=int, default=2000, help="served: number of requests")
+    ap.add_argument("--rate", type=float, default=1000.0, help="served: Poisson arrival rate (requests/s)")
+    ap.add_argument("--small-blocks", type=int, default=256 * 6, help="served: workgroups of the small-graph server")
+    ap.add_argument("--large-blocks", type=int, default=256, help="served: workgroups of the large-graph server")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--base-seed", type=int, default=1000)
     ap.add_argument("--max-nodes", type=int, default=1 << 20, help="per-graph GPU tree-node budget (batch)")
@@ -126,7 +244,9 @@ def main():
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded generators)"}
 
-    if args.workload == "batch":
+    if args.workload == "served":
+        served(ctx, args, out)
+    elif args.workload == "batch":
         graphs, meta = build_batch(args.count, args.base_seed)
         ubs = [vcgpu.upper_bound(g, seed=args.base_seed) for g in graphs]
         jobs = [vcgpu.Job(vcgpu.VC_MVC, 0, u, args.max_nodes) for u, _ in ubs]

@@ -106,6 +106,22 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t xu)
     return (uint32_t)(min(a, b) ^ (int)0x80000000);
 }
 
+// Inclusive prefix sum over the 64 lanes: DPP row_shr 1/2/3 on the input,
+// row_shr 4/8 with bank masks inside each row of 16, then row_bcast 15/31
+// to carry across rows (the GCN/CDNA scan sequence).
+__device__ __forceinline__ int wave_incl_scan(int x)
+{
+    int v = x;
+    v += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, x, 0x113, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xE, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xC, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, true);
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // HBM atomics (agent scope)
 // ---------------------------------------------------------------------------
@@ -166,6 +182,7 @@ struct Node {
     DegT *deg;           // [npad]: degree, -1 = in cover
     uint32_t *mark;      // [mwords] bitset, all zero between passes
     uint32_t *list;      // [lcap]: vertices marked in the current pass
+    int *xs;             // [2 * 64] scratch of apply_list: edge offsets, list bases
     const uint32_t *src; // graph srcPtr (HBM)
     const uint32_t *dst; // graph dst (HBM)
     uint32_t n;
@@ -200,35 +217,66 @@ __device__ __forceinline__ uint32_t append(const N &w, uint32_t L, uint32_t cand
     return min(L + (uint32_t)__popcll(m), w.lcap);
 }
 
+// Decrements a live vertex's degree.  Several lanes may hit the same vertex,
+// so the update is an atomic on the containing 32-bit word; for int16
+// degrees the half-word is addressed by adding -1 or -(1 << 16).  A decremented
+// vertex is live with degree >= 1, so no borrow crosses into the other half.
+template <class N>
+__device__ __forceinline__ void dec_degree(const N &w, uint32_t t)
+{
+    using D = typename N::deg_t;
+    if constexpr (sizeof(D) == 2) {
+        uint32_t *word = reinterpret_cast<uint32_t *>(w.deg) + (t >> 1);
+        uint32_t delta = (t & 1u) ? 0xFFFF0000u : 0xFFFFFFFFu;
+        __hip_atomic_fetch_add(word, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    } else {
+        __hip_atomic_fetch_add(reinterpret_cast<int32_t *>(w.deg) + t, -1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+}
+
 // Apply phase of a rule: every listed vertex goes into the cover.  First all
-// of them get -1, then each one decrements its neighbours that are still
-// live (src/helperFunctions.cuh:270-289).  Listed vertices are distinct and
-// live.  Neighbours of one vertex are distinct, so one vertex's decrements
-// need no atomics; vertices are taken one after the other.
+// of them get -1, then each decrements its neighbours that are still live
+// (src/helperFunctions.cuh:270-289).  Listed vertices are distinct and live.
+// The adjacency lists of up to 64 listed vertices are laid end to end over
+// the 64 lanes (wave prefix sum of their lengths; each lane finds the list
+// entry that owns its edge with a 6-step shuffle search), so one round of
+// global loads serves 64 edges whatever the degrees.
 template <class N>
 __device__ void apply_list(const N &w, uint32_t L)
 {
     w.sync();
     for (uint32_t i = lane_id(); i < L; i += kWave) w.deg[w.list[i]] = -1;
     w.sync();
+    const int lane = (int)lane_id();
+    int *const xoff = w.xs;          // exclusive edge offset of list entry j
+    int *const xbase = w.xs + kWave; // first adjacency index of list entry j
     for (uint32_t c0 = 0; c0 < L; c0 += kWave) {
-        uint32_t cnt = min(L - c0, (uint32_t)kWave);
-        uint32_t my_b = 0, my_e = 0;
-        if (lane_id() < cnt) {
-            uint32_t u = w.list[c0 + lane_id()];
-            my_b = w.src[u];
-            my_e = w.src[u + 1];
+        const uint32_t cnt = min(L - c0, (uint32_t)kWave);
+        int b = 0, len = 0;
+        if ((uint32_t)lane < cnt) {
+            const uint32_t u = w.list[c0 + lane];
+            b = (int)w.src[u];
+            len = (int)w.src[u + 1] - b;
         }
-        for (uint32_t j = 0; j < cnt; ++j) {
-            uint32_t b = __builtin_amdgcn_readlane(my_b, j);
-            uint32_t e = __builtin_amdgcn_readlane(my_e, j);
-            for (uint32_t x = b + lane_id(); x < e; x += kWave) {
-                uint32_t t = w.dst[x];
-                int dt = w.deg[t];
-                if (dt != -1) w.deg[t] = (typename N::deg_t)(dt - 1);
+        const int incl = wave_incl_scan(len);
+        xoff[lane] = incl - len;
+        xbase[lane] = b;
+        const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
+        w.sync();
+#pragma unroll 1
+        for (int x = lane; x < total; x += kWave) {
+            // owner: last entry j < cnt with xoff[j] <= x
+            int pos = 0;
+#pragma unroll 1
+            for (int step = 32; step >= 1; step >>= 1) {
+                const int cand = pos + step;
+                if (cand < (int)cnt && xoff[cand] <= x) pos = cand;
             }
-            w.sync();
+            const uint32_t t = w.dst[xbase[pos] + (x - xoff[pos])];
+            if (w.deg[t] != -1) dec_degree(w, t);
         }
+        w.sync();
     }
     for (uint32_t i = lane_id(); i < L; i += kWave) w.mark[w.list[i] >> 5] = 0u;
     w.sync();
@@ -639,6 +687,7 @@ __device__ __forceinline__ void flush_nodes(const SolveArgs &a, const GDesc &gd,
 // the request's pinned host area and lane 0 writes the completion record
 // (fields first, the sequence number last, with a system-scope release in
 // between so the host only ever sees complete records).
+template <bool SERVE>
 __device__ void finish_unit(const SolveArgs &a, uint32_t g, const GDesc &gd)
 {
     GState *gs = &a.gstate[g];
@@ -649,7 +698,10 @@ __device__ void finish_unit(const SolveArgs &a, uint32_t g, const GDesc &gd)
         last = old == 1u;
         if (last) st_rlx(&gs->done, 1u);
     }
-    if (!a.serving || !uni(last)) return;
+    if constexpr (!SERVE) {
+        return;
+    } else {
+    if (!uni(last)) return;
     fence_acq();
     const uint32_t found = uni(ld_rlx(&gs->found));
     const int32_t cs = (int32_t)uni((uint32_t)ld_rlx(&gs->cover_size));
@@ -675,6 +727,7 @@ __device__ void finish_unit(const SolveArgs &a, uint32_t g, const GDesc &gd)
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
     if (lane_id() == 0) __hip_atomic_store(&a.comp[g].seq, gd.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // Serving: the worker that claimed request slot g brings it into HBM -- the
@@ -712,9 +765,10 @@ __device__ void load_request(const SolveArgs &a, uint32_t g)
 // Number of roots that may be handed out.  Batch: all of them.  Serving: the
 // host's request tail, polled at most once per poll interval by one worker
 // (system-scope acquire on pinned memory) and shared through a device copy.
+template <bool SERVE>
 __device__ __forceinline__ uint32_t roots_available(const SolveArgs &a)
 {
-    if (!a.serving) return a.roots;
+    if constexpr (!SERVE) return a.roots;
     uint32_t t = 0;
     if (lane_id() == 0) {
         t = ld_rlx(&a.glob->host_tail);
@@ -741,7 +795,7 @@ struct Acquired {
     uint32_t kind, g, d, v, root;
 };
 
-template <class N>
+template <bool SERVE, class N>
 __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
 {
     Acquired r = {0u, kNone, 0u, kNone, 0u};
@@ -758,12 +812,12 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
                 return r;
             }
         }
-        const uint32_t avail = roots_available(a);
+        const uint32_t avail = roots_available<SERVE>(a);
         if (uni(ld_rlx(&a.glob->next_root)) < avail) {
             uint32_t idx = kNone;
             if (lane_id() == 0) {
                 fetch_add(&a.glob->counter, (unsigned long long)(0ull - kWaitOne));
-                if (!a.serving) {
+                if constexpr (!SERVE) {
                     // batch: indices past the fixed root count are simply invalid
                     idx = fetch_add(&a.glob->next_root, 1u);
                 } else {
@@ -784,7 +838,7 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
             if (idx < avail) {
                 r.kind = 2;
                 r.root = idx;
-                if (a.serving) {
+                if constexpr (SERVE) {
                     // the host wrote the ring entry, descriptor and graph to
                     // pinned memory before publishing the tail
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
@@ -802,7 +856,7 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
         if (lane_id() == 0) c = ld_rlx(&a.glob->counter);
         uint32_t lo = uni((uint32_t)c), hi = uni((uint32_t)(c >> 32));
         uint32_t waiting = lo & 0xFFFFu, reg = lo >> 16;
-        const bool no_more_roots = a.serving ? (uni(ld_rlx(&a.glob->host_shutdown)) != 0u &&
+        const bool no_more_roots = SERVE ? (uni(ld_rlx(&a.glob->host_shutdown)) != 0u &&
                                                 uni(ld_rlx(&a.glob->next_root)) >= avail)
                                              : uni(ld_rlx(&a.glob->next_root)) >= a.roots;
         if (waiting == reg && hi == 0u && no_more_roots) {
@@ -825,15 +879,15 @@ __device__ __forceinline__ long long sq64(int x) { return (long long)x * (long l
 
 constexpr uint32_t kFlushEvery = 1024;
 
-template <typename DegT, bool HBM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void solve_kernel(SolveArgs a)
+template <typename DegT, bool HBM, bool SERVE>
+__global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
 {
     extern __shared__ int4 smem4[];
     const uint32_t wib = threadIdx.x / kWave;
     const uint32_t wid = blockIdx.x * (blockDim.x / kWave) + wib;
     if (wid >= a.workers) return;
 
-    // node storage: deg [npad DegT] | list [lcap u32] | mark [mwords u32]
+    // node storage: deg [npad DegT] | list [lcap u32] | mark [mwords u32] | xs [128 i32]
     char *store = HBM ? a.hbm_nodes + (size_t)wid * a.worker_bytes
                       : reinterpret_cast<char *>(smem4) + (size_t)wib * a.worker_bytes;
     const uint32_t slot_bytes = a.npad * (uint32_t)sizeof(DegT);
@@ -841,6 +895,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     w.deg = reinterpret_cast<DegT *>(store);
     w.list = reinterpret_cast<uint32_t *>(store + slot_bytes);
     w.mark = w.list + a.lcap;
+    w.xs = reinterpret_cast<int *>(w.mark + a.mwords);
     w.src = nullptr;
     w.dst = nullptr;
     w.n = 0;
@@ -882,12 +937,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 if (g != kNone) {
                     total_nodes += pending;
                     flush_nodes(a, gd, g, pending);
-                    finish_unit(a, g, gd);
+                    finish_unit<SERVE>(a, g, gd);
                     g = kNone;
                 }
                 if (lane_id() == 0 && ld_rlx(&a.glob->cancel)) st_rlx(&a.glob->done_flag, 1u);
                 const uint64_t t_idle = wall_clock64();
-                Acquired q = acquire(a, w, slot_bytes);
+                Acquired q = acquire<SERVE>(a, w, slot_bytes);
                 idle_ticks += wall_clock64() - t_idle;
                 if (q.kind == 0) break;
                 steals += q.kind == 1;
@@ -1066,9 +1121,12 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
     }
 }
 
-template __global__ void solve_kernel<int16_t, false>(SolveArgs);
-template __global__ void solve_kernel<int32_t, false>(SolveArgs);
-template __global__ void solve_kernel<int32_t, true>(SolveArgs);
+template __global__ void solve_kernel<int16_t, false, false>(SolveArgs);
+template __global__ void solve_kernel<int32_t, false, false>(SolveArgs);
+template __global__ void solve_kernel<int32_t, true, false>(SolveArgs);
+template __global__ void solve_kernel<int16_t, false, true>(SolveArgs);
+template __global__ void solve_kernel<int32_t, false, true>(SolveArgs);
+template __global__ void solve_kernel<int32_t, true, true>(SolveArgs);
 
 }  // namespace vcg
 
@@ -1151,12 +1209,19 @@ bool csr_is_valid(const vc_csr &g)
 // Kernel variants: degree type x node storage.
 enum Variant { kLds16 = 0, kLds32 = 1, kHbm32 = 2 };
 
-const void *kernel_of(int variant)
+const void *kernel_of(int variant, bool serve)
 {
+    if (serve) {
+        switch (variant) {
+        case kLds16: return reinterpret_cast<const void *>(vcg::solve_kernel<int16_t, false, true>);
+        case kLds32: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, false, true>);
+        default: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, true, true>);
+        }
+    }
     switch (variant) {
-    case kLds16: return reinterpret_cast<const void *>(vcg::solve_kernel<int16_t, false>);
-    case kLds32: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, false>);
-    default: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, true>);
+    case kLds16: return reinterpret_cast<const void *>(vcg::solve_kernel<int16_t, false, false>);
+    case kLds32: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, false, false>);
+    default: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, true, false>);
     }
 }
 
@@ -1168,19 +1233,19 @@ struct Plan {
     size_t off_glob, off_stk, off_meta, off_qdeg, off_qmeta, off_tickets, off_wnodes, off_hbm, total;
 };
 
-// Per-worker node storage for a degree type: deg [npad] | list [lcap] | mark [mwords].
+// Per-worker node storage for a degree type: deg [npad] | list [lcap] | mark [mwords] | xs [128].
 void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes)
 {
     pl.deg_bytes = deg_bytes;
     pl.npad = round_up(nmax, 64u / deg_bytes);
     pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), 1024u);
     pl.mwords = round_up((nmax + 31u) / 32u, 4u);
-    pl.worker_bytes = round_up(pl.npad * deg_bytes + 4u * pl.lcap + 4u * pl.mwords, 64u);
+    pl.worker_bytes = round_up(pl.npad * deg_bytes + 4u * pl.lcap + 4u * pl.mwords + 4u * 2u * vcg::kWave, 64u);
 }
 
 // Launch shape shared by the batch solve and the server: node storage,
 // workgroup, grid from the occupancy query, worklist, and the memory plan.
-vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_params *p, Plan &pl)
+vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_params *p, Plan &pl, bool serve)
 {
     const uint32_t version = p->version;
     if (version != VC_HYBRID && version != VC_STACK_ONLY) return VC_EINVAL;
@@ -1211,7 +1276,8 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
     if (pl.smem > lds_limit) return VC_ENOTSUP;
 
     int bpc = 0;
-    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel_of(pl.variant), (int)pl.block_dim, pl.smem)))
+    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel_of(pl.variant, serve), (int)pl.block_dim,
+                                                             pl.smem)))
         return VC_EDEVICE;
     if (bpc < 1) return VC_ENOTSUP;
     pl.blocks_per_cu = (uint32_t)bpc;
@@ -1275,7 +1341,7 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
         lim = std::min(lim, set->h_desc[i].n);
         need = std::max(need, lim);
     }
-    vc_status st = plan_launch(ctx, set->nmax, need + 1u, p, pl);
+    vc_status st = plan_launch(ctx, set->nmax, need + 1u, p, pl, false);
     if (st != VC_OK) return st;
     pl.roots = (uint32_t)set->count * pl.units;
     return VC_OK;
@@ -1306,21 +1372,31 @@ void fill_args(SolveArgs &a, const Plan &pl, char *base, uint32_t version, uint3
     a.hbm_nodes = pl.variant == kHbm32 ? base + pl.off_hbm : nullptr;
 }
 
-bool launch_solve(const Plan &pl, hipStream_t stream, const SolveArgs &a)
+template <bool SERVE>
+void launch_variant(const Plan &pl, hipStream_t stream, const SolveArgs &a)
 {
     switch (pl.variant) {
     case kLds16:
-        hipLaunchKernelGGL((vcg::solve_kernel<int16_t, false>), dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem,
-                           stream, a);
+        hipLaunchKernelGGL((vcg::solve_kernel<int16_t, false, SERVE>), dim3(pl.num_blocks), dim3(pl.block_dim),
+                           pl.smem, stream, a);
         break;
     case kLds32:
-        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, false>), dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem,
-                           stream, a);
+        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, false, SERVE>), dim3(pl.num_blocks), dim3(pl.block_dim),
+                           pl.smem, stream, a);
         break;
     default:
-        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, true>), dim3(pl.num_blocks), dim3(pl.block_dim), 0, stream, a);
+        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, true, SERVE>), dim3(pl.num_blocks), dim3(pl.block_dim), 0,
+                           stream, a);
         break;
     }
+}
+
+bool launch_solve(const Plan &pl, hipStream_t stream, const SolveArgs &a)
+{
+    if (a.serving)
+        launch_variant<true>(pl, stream, a);
+    else
+        launch_variant<false>(pl, stream, a);
     return hip_ok(hipGetLastError());
 }
 
@@ -1930,7 +2006,7 @@ vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **
     p.num_blocks = cfg->num_blocks;
     p.use_global_memory = -1;
     const uint32_t depth = cfg->stack_depth ? cfg->stack_depth : cfg->max_vertices + 1u;
-    vc_status st = plan_launch(ctx, cfg->max_vertices, depth, &p, s->pl);
+    vc_status st = plan_launch(ctx, cfg->max_vertices, depth, &p, s->pl, true);
     if (st != VC_OK) {
         delete s;
         return st;

@@ -18,7 +18,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libvcgpu.so")
+# VCGPU_LIBRARY selects another build of the same library (A/B measurements).
+LIB_PATH = os.environ.get("VCGPU_LIBRARY") or os.path.join(_HERE, "libvcgpu.so")
 
 VC_OK, VC_EINVAL, VC_ENOMEM, VC_EDEVICE, VC_ETIMEOUT, VC_ECANCELLED, VC_ENOTSUP, VC_EBADGRAPH = range(8)
 VC_MVC, VC_PVC = 0, 1
