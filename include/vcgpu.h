@@ -76,8 +76,13 @@ typedef struct {
     uint32_t time_limit_ms;     /* wall-clock limit of the launch; 0 = one hour.  When it
                                    passes every worker drains and exits, and unfinished
                                    graphs report VC_ETIMEOUT. */
-    uint32_t flags;             /* reserved, 0 */
+    uint32_t flags;             /* VC_FLAG_* */
 } vc_params;
+
+/* vc_params.flags: run the solve as a HIP graph (init kernel, persistent
+ * solve kernel and result copies captured once per graph set and launch
+ * shape, replayed on later calls with the same shape). */
+#define VC_FLAG_GRAPH 1u
 
 /* Per-graph outcome.  Replaces the 4-byte minimum_d / kFound_d readback of
  * src/main.cu:355-359 and adds the node count and status. */
@@ -144,6 +149,17 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const 
  * cudaOccupancyMaxActiveBlocksPerMultiprocessor, src/main.cu:131-191). */
 vc_status vc_occupancy(vc_ctx *ctx, const vc_graphs *set, const vc_params *params, int *blocks_per_cu,
                        int *block_dim, int *use_global_memory);
+
+/* Launch-shape autotuner (replaces the reference's fixed search,
+ * setBlockDimAndUseGlobalMemoryAutoWorkListSize, src/auxFunctions.cpp:469-578):
+ * times candidate shapes -- workgroup size, LDS or HBM node storage,
+ * worklist size -- by solving `set` with `jobs`, and writes the fastest into
+ * *best (other fields copied from *base).  Results are cached in the
+ * context by (mean vertex count, mean density) bucket, so later sets of the
+ * same kind are not re-timed.  *best_ms (optional) gets the winning kernel
+ * time (0 when served from the cache). */
+vc_status vc_autotune(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *base,
+                      vc_params *best, float *best_ms);
 
 /* ---- serving: continuous batching over one persistent launch ----
  *

@@ -1135,8 +1135,10 @@ template __global__ void solve_kernel<int32_t, true, true>(SolveArgs);
 // ===========================================================================
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "vcgpu.h"
@@ -1157,6 +1159,18 @@ struct vc_ctx {
     // launch scratch, grown on demand and reused across solves
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // autotuner cache: (vertex-count bucket, density bucket) -> chosen params
+    std::map<std::pair<int, int>, vc_params> tuned;
+};
+
+// What a captured dispatch depends on; a different value means re-capture.
+struct DispatchKey {
+    int variant;
+    uint32_t num_blocks, block_dim, version, start_depth, qsize, qthreshold, depth, want_covers;
+    size_t smem;
+    uint64_t limit_ticks;
+    const void *scratch;
+    const void *h_ws;
 };
 
 struct vc_graphs {
@@ -1168,6 +1182,18 @@ struct vc_graphs {
     std::vector<GDesc> h_desc;
     uint32_t nmax = 0;
     uint32_t cover_words = 0;
+    double mean_n = 0, mean_density = 0;  // autotuner bucket
+    // pinned readback buffers (stable addresses, so a captured dispatch can
+    // copy into them on every replay)
+    GState *h_state = nullptr;
+    Global *h_glob = nullptr;
+    vcg::WorkerStats *h_ws = nullptr;
+    uint32_t ws_cap = 0;
+    uint32_t *h_cov = nullptr;
+    // captured dispatch: init kernel + solve kernel + readback copies
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DispatchKey key{};
 };
 
 namespace {
@@ -1495,6 +1521,11 @@ vc_status vc_graphs_upload(vc_ctx *ctx, int count, const vc_csr *graphs, vc_grap
         }
     }
     s->cover_words = std::max<uint32_t>(cover_words, 2u);
+    for (int i = 0; i < count; ++i) {
+        const double n = graphs[i].vertexNum, m = graphs[i].edgeNum;
+        s->mean_n += n / count;
+        s->mean_density += (n > 1 ? 2.0 * m / (n * (n - 1)) : 0.0) / count;
+    }
     std::vector<uint32_t> host(std::max<size_t>(words, 4), 0u);
     for (int i = 0; i < count; ++i) {
         const vc_csr &g = graphs[i];
@@ -1530,6 +1561,12 @@ vc_status vc_graphs_free(vc_ctx *ctx, vc_graphs *s)
     if (s->d_desc) hipFree(s->d_desc);
     if (s->d_covers) hipFree(s->d_covers);
     if (s->d_state) hipFree(s->d_state);
+    if (s->h_state) hipHostFree(s->h_state);
+    if (s->h_glob) hipHostFree(s->h_glob);
+    if (s->h_ws) hipHostFree(s->h_ws);
+    if (s->h_cov) hipHostFree(s->h_cov);
+    if (s->exec) hipGraphExecDestroy(s->exec);
+    if (s->graph) hipGraphDestroy(s->graph);
     delete s;
     return VC_OK;
 }
@@ -1605,35 +1642,96 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         fill_args(a, pl, base, params->version, params->start_depth, ms * (uint64_t)ctx->wall_khz);
     }
 
-    hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, ctx->stream, a, pl.units);
-    if (!hip_ok(hipGetLastError())) return VC_EDEVICE;
-    hipEventRecord(ctx->ev0, ctx->stream);
-    if (!launch_solve(pl, ctx->stream, a)) return VC_EDEVICE;
-    hipEventRecord(ctx->ev1, ctx->stream);
-
-    std::vector<GState> hs(set->count);
-    Global hg;
-    std::vector<vcg::WorkerStats> wn(pl.workers);
-    bool ok = hip_ok(hipMemcpyAsync(hs.data(), set->d_state, sizeof(GState) * set->count, hipMemcpyDeviceToHost,
-                                    ctx->stream)) &&
-              hip_ok(hipMemcpyAsync(&hg, a.glob, sizeof(Global), hipMemcpyDeviceToHost, ctx->stream)) &&
-              hip_ok(hipMemcpyAsync(wn.data(), a.wstats, sizeof(vcg::WorkerStats) * pl.workers, hipMemcpyDeviceToHost,
-                                    ctx->stream));
-    std::vector<uint32_t> bitmaps;
     bool want_covers = false;
     if (covers)
         for (int i = 0; i < set->count; ++i) want_covers |= covers[i] != nullptr;
-    if (ok && want_covers) {
-        bitmaps.resize(set->cover_words);
-        ok = hip_ok(hipMemcpyAsync(bitmaps.data(), set->d_covers, sizeof(uint32_t) * set->cover_words,
-                                   hipMemcpyDeviceToHost, ctx->stream));
+    // pinned readback buffers
+    bool ok = true;
+    if (!set->h_state)
+        ok = hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_state), sizeof(GState) * set->count)) &&
+             hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_glob), sizeof(Global))) &&
+             hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_cov), sizeof(uint32_t) * set->cover_words));
+    if (ok && set->ws_cap < pl.workers) {
+        if (set->h_ws) hipHostFree(set->h_ws);
+        set->h_ws = nullptr;
+        ok = hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_ws), sizeof(vcg::WorkerStats) * pl.workers));
+        set->ws_cap = ok ? pl.workers : 0;
     }
-    if (!ok || !hip_ok(hipStreamSynchronize(ctx->stream))) return VC_EDEVICE;
+    if (!ok) return VC_ENOMEM;
+
+    // the device work of one solve, in stream order
+    auto enqueue_dispatch = [&](bool timed) -> bool {
+        hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, ctx->stream, a, pl.units);
+        if (!hip_ok(hipGetLastError())) return false;
+        if (timed && !hip_ok(hipEventRecord(ctx->ev0, ctx->stream))) return false;
+        if (!launch_solve(pl, ctx->stream, a)) return false;
+        if (timed && !hip_ok(hipEventRecord(ctx->ev1, ctx->stream))) return false;
+        bool r = hip_ok(hipMemcpyAsync(set->h_state, set->d_state, sizeof(GState) * set->count, hipMemcpyDeviceToHost,
+                                       ctx->stream)) &&
+                 hip_ok(hipMemcpyAsync(set->h_glob, a.glob, sizeof(Global), hipMemcpyDeviceToHost, ctx->stream)) &&
+                 hip_ok(hipMemcpyAsync(set->h_ws, a.wstats, sizeof(vcg::WorkerStats) * pl.workers,
+                                       hipMemcpyDeviceToHost, ctx->stream));
+        if (r && want_covers)
+            r = hip_ok(hipMemcpyAsync(set->h_cov, set->d_covers, sizeof(uint32_t) * set->cover_words,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+        return r;
+    };
+
+    if (params->flags & VC_FLAG_GRAPH) {
+        // HIP graph: capture once per (set, launch shape), replay afterwards.
+        // The solve is bracketed by events outside the graph.
+        DispatchKey key;
+        std::memset(&key, 0, sizeof(key));
+        key.variant = pl.variant;
+        key.num_blocks = pl.num_blocks;
+        key.block_dim = pl.block_dim;
+        key.version = a.version;
+        key.start_depth = a.start_depth;
+        key.qsize = pl.qsize;
+        key.qthreshold = pl.qthreshold;
+        key.depth = pl.depth;
+        key.want_covers = want_covers;
+        key.smem = pl.smem;
+        key.limit_ticks = a.limit_ticks;
+        key.scratch = ctx->scratch;
+        key.h_ws = set->h_ws;
+        if (!set->exec || std::memcmp(&key, &set->key, sizeof(key)) != 0) {
+            if (set->exec) hipGraphExecDestroy(set->exec);
+            if (set->graph) hipGraphDestroy(set->graph);
+            set->exec = nullptr;
+            set->graph = nullptr;
+            if (!hip_ok(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal))) return VC_EDEVICE;
+            bool cap = enqueue_dispatch(false);
+            hipGraph_t g = nullptr;
+            bool ended = hip_ok(hipStreamEndCapture(ctx->stream, &g));
+            if (!cap || !ended || !g) {
+                if (g) hipGraphDestroy(g);
+                return VC_EDEVICE;
+            }
+            if (!hip_ok(hipGraphInstantiate(&set->exec, g, nullptr, nullptr, 0))) {
+                hipGraphDestroy(g);
+                return VC_EDEVICE;
+            }
+            set->graph = g;
+            set->key = key;
+        }
+        if (!hip_ok(hipEventRecord(ctx->ev0, ctx->stream)) || !hip_ok(hipGraphLaunch(set->exec, ctx->stream)) ||
+            !hip_ok(hipEventRecord(ctx->ev1, ctx->stream)))
+            return VC_EDEVICE;
+    } else {
+        if (!enqueue_dispatch(true)) return VC_EDEVICE;
+    }
+    if (!hip_ok(hipStreamSynchronize(ctx->stream))) return VC_EDEVICE;
     float ms = 0.f;
     hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    const GState *hs = set->h_state;
+    const Global &hg = *set->h_glob;
+    const vcg::WorkerStats *wn = set->h_ws;
+    const uint32_t *bitmaps = set->h_cov;
 
     uint64_t total_nodes = 0, idle = 0, donations = 0, steals = 0;
-    for (const vcg::WorkerStats &x : wn) {
+    for (uint32_t wi = 0; wi < pl.workers; ++wi) {
+        const vcg::WorkerStats &x = wn[wi];
         total_nodes += x.nodes;
         idle += x.idle_ticks;
         donations += x.donations;
@@ -1685,6 +1783,73 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// autotuner
+// ===========================================================================
+
+extern "C" vc_status vc_autotune(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *base,
+                                 vc_params *best, float *best_ms)
+{
+    if (!ctx || !set || !jobs || !base || !best) return VC_EINVAL;
+    // bucket: log2 of the mean vertex count, log2 of the mean density in half steps
+    const int nb = (int)std::floor(std::log2(std::max(1.0, set->mean_n)));
+    const int db = (int)std::floor(std::log2(std::max(1e-6, set->mean_density)) * 2.0);
+    const std::pair<int, int> key(nb, db);
+    auto hit = ctx->tuned.find(key);
+    if (hit != ctx->tuned.end()) {
+        *best = hit->second;
+        best->time_limit_ms = base->time_limit_ms;
+        best->flags = base->flags;
+        if (best_ms) *best_ms = 0.f;
+        return VC_OK;
+    }
+    // candidates: workgroup of 1, 2 or 4 worker waves; node storage in LDS or
+    // HBM; worklist sized for the grid or 4x larger
+    std::vector<vc_params> cands;
+    for (uint32_t bd : {64u, 128u, 256u})
+        for (int gm : {0, 1})
+            for (uint32_t qmul : {0u, 4u}) {
+                vc_params p = *base;
+                p.block_dim = bd;
+                p.use_global_memory = gm;
+                p.list_size = 0;
+                p.list_threshold = 0.f;
+                p.flags = 0;
+                if (qmul) {
+                    Plan pl;
+                    if (make_plan(ctx, set, jobs, &p, pl) != VC_OK) continue;
+                    p.list_size = next_pow2(pl.qsize * qmul);
+                }
+                cands.push_back(p);
+            }
+    std::vector<vc_result> res(set->count);
+    float best_t = 0.f;
+    int best_i = -1;
+    for (size_t i = 0; i < cands.size(); ++i) {
+        Plan pl;
+        if (make_plan(ctx, set, jobs, &cands[i], pl) != VC_OK) continue;
+        vc_launch_info info;
+        vc_status st = vc_solve(ctx, set, jobs, &cands[i], res.data(), nullptr, &info);
+        if (st != VC_OK && st != VC_ETIMEOUT) continue;
+        // time per solved graph, so a shape that finishes more graphs within
+        // the jobs' node budgets is not penalised for it
+        uint32_t solved = 0;
+        for (const vc_result &r : res) solved += r.solved;
+        const float t = info.kernel_ms / (float)std::max<uint32_t>(solved, 1u);
+        if (best_i < 0 || t < best_t) {
+            best_t = t;
+            best_i = (int)i;
+        }
+    }
+    if (best_i < 0) return VC_ENOTSUP;
+    *best = cands[best_i];
+    best->time_limit_ms = base->time_limit_ms;
+    best->flags = base->flags;
+    ctx->tuned[key] = *best;
+    if (best_ms) *best_ms = best_t;
+    return VC_OK;
+}
 
 // ===========================================================================
 // serving: continuous batching over one persistent launch

@@ -75,7 +75,7 @@ EXPORTS = [
     "vc_ctx_create", "vc_ctx_destroy", "vc_graphs_upload", "vc_graphs_free", "vc_solve", "vc_occupancy",
     "vc_load_graph_file", "vc_csr_from_edges", "vc_csr_free", "vc_check_graph", "vc_approx_remove_max",
     "vc_approx_remove_edge", "vc_status_string", "vc_server_start", "vc_server_submit", "vc_server_poll",
-    "vc_server_wait", "vc_server_stop", "vc_now_ns",
+    "vc_server_wait", "vc_server_stop", "vc_now_ns", "vc_autotune",
 ]
 VC_EAGAIN = 8
 
@@ -104,6 +104,8 @@ def lib():
         L.vc_solve.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_Job), C.POINTER(_Params), C.POINTER(_Result),
                                C.POINTER(_u8p), C.POINTER(_Info)]
         L.vc_occupancy.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_Params), _i32p, _i32p, _i32p]
+        L.vc_autotune.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_Job), C.POINTER(_Params), C.POINTER(_Params),
+                                  C.POINTER(C.c_float)]
         L.vc_load_graph_file.argtypes = [C.c_char_p, C.POINTER(_CsrOwned)]
         L.vc_csr_from_edges.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p, C.POINTER(_CsrOwned)]
         L.vc_csr_free.argtypes = [C.POINTER(_CsrOwned)]
@@ -259,10 +261,19 @@ class Params:
     list_threshold: float = 0.0
     start_depth: int = 4
     time_limit_ms: int = 0
+    flags: int = 0
 
     def _c(self) -> _Params:
         return _Params(self.version, self.block_dim, self.num_blocks, self.use_global_memory, self.list_size,
-                       self.list_threshold, self.start_depth, self.time_limit_ms, 0)
+                       self.list_threshold, self.start_depth, self.time_limit_ms, self.flags)
+
+    @staticmethod
+    def _from_c(p: "_Params") -> "Params":
+        return Params(p.version, p.block_dim, p.num_blocks, p.use_global_memory, p.list_size, p.list_threshold,
+                      p.start_depth, p.time_limit_ms, p.flags)
+
+
+VC_FLAG_GRAPH = 1
 
 
 class GraphSet:
@@ -321,6 +332,18 @@ class Context:
         _check(lib().vc_occupancy(self.handle, gset.handle, C.byref(p), C.byref(b), C.byref(d), C.byref(g)),
                "vc_occupancy")
         return b.value, d.value, g.value
+
+    def autotune(self, gset: GraphSet, jobs, base: Params | None = None):
+        """vc_autotune: (best Params, best kernel ms per solved graph or 0 if cached)."""
+        G = len(gset)
+        if isinstance(jobs, Job):
+            jobs = [jobs] * G
+        cj = (_Job * G)(*[_Job(j.instance, j.k, j.upper_bound, 0, j.max_nodes) for j in jobs])
+        b = (base or Params())._c()
+        out = _Params()
+        ms = C.c_float()
+        _check(lib().vc_autotune(self.handle, gset.handle, cj, C.byref(b), C.byref(out), C.byref(ms)), "vc_autotune")
+        return Params._from_c(out), float(ms.value)
 
     def solve(self, gset: GraphSet, jobs, params: Params | None = None, want_covers: bool = True,
               raise_on_error: bool = True):

@@ -204,3 +204,39 @@ def test_reference_known_answers_hard(ctx, case):
         assert r.minimum == meta["mvc"]
     print(f"{name}: solved={r.solved} minimum={r.minimum} known={meta['mvc']} nodes={r.tree_nodes} "
           f"kernel_ms={info.kernel_ms:.0f}")
+
+
+def test_graph_captured_dispatch_replays_match_eager(ctx):
+    """VC_FLAG_GRAPH: the captured dispatch replayed several times, with eager
+    solves in between, gives the eager answers on every replay."""
+    cases = _medium_cases()[:6] + [(n, 30, graphgen.erdos_renyi(30, 0.2, 77)) for n in ["er30"]]
+    graphs = [vcgpu.csr_from_edges(n, e) for _, n, e in cases]
+    jobs = [vcgpu.Job(vcgpu.VC_MVC, 0, vcgpu.upper_bound(g, seed=2)[0], 0) for g in graphs]
+    gs = ctx.upload(graphs)
+    eager, _, st = ctx.solve(gs, jobs, P())
+    assert st == vcgpu.VC_OK
+    for rep in range(4):
+        res, info, st = ctx.solve(gs, jobs, P(flags=vcgpu.VC_FLAG_GRAPH))
+        assert st == vcgpu.VC_OK
+        for (name, n, e), a, b in zip(cases, eager, res):
+            assert b.solved and b.minimum == a.minimum, (rep, name)
+            cover = b.cover
+            if cover is not None:
+                og = O.from_edges(n, e)
+                assert O.cover_is_valid(og, np.asarray(cover, np.uint8)) and int(np.sum(cover)) == b.minimum
+        again, _, _ = ctx.solve(gs, jobs, P())  # eager work between replays
+        assert [r.minimum for r in again] == [r.minimum for r in eager]
+
+
+def test_autotuner_picks_a_working_shape(ctx):
+    cases = _medium_cases()[:8]
+    graphs = [vcgpu.csr_from_edges(n, e) for _, n, e in cases]
+    jobs = [vcgpu.Job(vcgpu.VC_MVC, 0, vcgpu.upper_bound(g, seed=2)[0], 0) for g in graphs]
+    gs = ctx.upload(graphs)
+    best, ms = ctx.autotune(gs, jobs, P())
+    assert best.block_dim in (64, 128, 256) and best.use_global_memory in (0, 1)
+    again, ms2 = ctx.autotune(gs, jobs, P())
+    assert ms2 == 0.0 and again.block_dim == best.block_dim  # cached by bucket
+    res, info, st = ctx.solve(gs, jobs, best)
+    for (name, n, e), r in zip(cases, res):
+        assert r.solved and r.minimum == O.mvc(O.from_edges(n, e))[0], name
