@@ -50,7 +50,7 @@ void usage()
                  "   -t <Global Work List Threshold> donate while queued < t * size, 0.0-1.0 (default: half the workers)\n"
                  "   -d <Starting Depth>           static split depth for stackonly (default 4)\n"
                  "   -b <Block Dimension>          threads per workgroup, multiple of 64, <= 256 (default 256)\n"
-                 "   -g <Use Global Memory>        0 = degree arrays in LDS (1 is not supported in this build)\n"
+                 "   -g <Use Global Memory>        0 = node storage in LDS, 1 = in HBM (default: LDS when it fits)\n"
                  "   -n <Number of Blocks>         workgroups (default occupancy x CUs)\n\n"
                  "   VC_SEED=<u32> fixes the RemoveEdge seed (default time(0), as the reference).\n\n");
 }
@@ -170,11 +170,6 @@ int main(int argc, char **argv)
 
     if (c.version == SEQUENTIAL) {
         std::fprintf(stderr, "-v sequential is the reference's CPU solver and is not part of this GPU build\n");
-        vc_csr_free(&owned);
-        return 2;
-    }
-    if (c.userDefMemory && c.useGlobalMemory) {
-        std::fprintf(stderr, "-g 1 (degree arrays in HBM) is not supported by this build\n");
         vc_csr_free(&owned);
         return 2;
     }
