@@ -94,55 +94,72 @@ def cpu_baseline_single(n, e, budget_s: float):
 
 
 def served(ctx, args, out):
-    """Served mixed stream: Poisson arrivals of MVC / PVC(k = opt +- 1)
-    requests, 95% graphs of the batch distribution and 5% Barabasi-Albert
-    n=20000 m=4, through two continuous-batching servers (small and large
-    graph classes) that share the GPU.  opt is precomputed offline by a batch
-    solve of the same graphs (the "alone" answers); graphs whose opt that
-    precompute cannot establish within its budget are not drawn, since the
-    PVC requests need opt."""
+    """Served mixed stream: Poisson arrivals of MVC / PVC requests through
+    two continuous-batching servers (small and large graph classes) that
+    share the GPU.
+
+    Small class (95%): graphs of the batch distribution, 50/50 MVC or PVC at
+    k = opt +- 1, with opt from an offline batch solve of the same graphs (the
+    "alone" answers); graphs whose opt that precompute cannot establish within
+    its budget are not drawn, since their PVC requests need opt.
+
+    Large class (5%): Barabasi-Albert n=20000 m=4.  Their exact MVC is far
+    out of reach of this rule set (U ~ 10,800 and a search tree thousands of
+    levels deep), so opt is unknown: they are served with a per-request node
+    budget, 50/50 as MVC (returns an incumbent, counted unsolved when the
+    budget ends the search) and PVC at k = U, whose answer is "yes" (the
+    approximation's cover has size U); a "yes" is only counted when its cover
+    checks.  `value` counts exactly answered requests of both classes."""
     import threading
     rng = np.random.Generator(np.random.PCG64(args.base_seed + 77))
     n_req = args.requests
     n_large = max(1, n_req // 20)
-    # candidate pool and offline opt
+    # candidate pool and offline opt (small class)
     small_cands = [graphgen.batch_graph(i, args.base_seed)[:2] for i in range(min(3 * n_req, 4096))]
     large_cands = [(20000, graphgen.barabasi_albert(20000, 4, 5000 + j)) for j in range(min(n_large, 8))]
     pool = [vcgpu.csr_from_edges(n, e) for n, e in small_cands + large_cands]
+    large_edges = {len(small_cands) + j: e for j, (_, e) in enumerate(large_cands)}
     ubs = [vcgpu.upper_bound(g, seed=args.base_seed) for g in pool]
     t0 = time.perf_counter()
     res_small, _, _ = ctx.solve(ctx.upload(pool[:len(small_cands)]),
                                 [vcgpu.Job(vcgpu.VC_MVC, 0, u, 1 << 18) for u, _ in ubs[:len(small_cands)]],
                                 vcgpu.Params(time_limit_ms=args.time_limit_ms), want_covers=False,
                                 raise_on_error=False)
-    res_large, _, _ = ctx.solve(ctx.upload(pool[len(small_cands):]),
-                                [vcgpu.Job(vcgpu.VC_MVC, 0, u, 1 << 18) for u, _ in ubs[len(small_cands):]],
-                                vcgpu.Params(time_limit_ms=args.time_limit_ms), want_covers=False,
-                                raise_on_error=False)
     precompute_s = time.perf_counter() - t0
-    opts = [r.minimum if r.solved else None for r in list(res_small) + list(res_large)]
+    opts = [r.minimum if r.solved else None for r in res_small]
     small_ok = [i for i in range(len(small_cands)) if opts[i] is not None]
-    large_ok = [len(small_cands) + j for j in range(len(large_cands)) if opts[len(small_cands) + j] is not None]
-    # the stream
+    large_ids = list(range(len(small_cands), len(pool)))
+    # the stream: (graph, job, expected answer or None, class)
     reqs = []
     for r in range(n_req):
-        big = large_ok and rng.random() < 0.05
-        gi = int(rng.choice(large_ok)) if big else int(small_ok[int(rng.integers(len(small_ok)))])
+        big = rng.random() < 0.05
+        mvc = rng.random() < 0.5
+        if big:
+            gi = int(rng.choice(large_ids))
+            U = ubs[gi][0]
+            if mvc:
+                reqs.append((gi, vcgpu.Job(vcgpu.VC_MVC, 0, U, args.large_budget), None, "large"))
+            else:
+                reqs.append((gi, vcgpu.Job(vcgpu.VC_PVC, U, U, args.large_budget), ("pvc", 1), "large"))
+            continue
+        gi = int(small_ok[int(rng.integers(len(small_ok)))])
         opt = opts[gi]
-        if rng.random() < 0.5:
-            job = vcgpu.Job(vcgpu.VC_MVC, 0, ubs[gi][0], 0)
-            want = ("mvc", opt)
+        if mvc:
+            reqs.append((gi, vcgpu.Job(vcgpu.VC_MVC, 0, ubs[gi][0], 0), ("mvc", opt), "small"))
         else:
             k = max(0, opt + (1 if rng.random() < 0.5 else -1))
-            job = vcgpu.Job(vcgpu.VC_PVC, k, ubs[gi][0], 0)
-            want = ("pvc", int(k >= opt))
-        reqs.append((gi, job, want))
+            reqs.append((gi, vcgpu.Job(vcgpu.VC_PVC, k, ubs[gi][0], 0), ("pvc", int(k >= opt)), "small"))
     gaps = rng.exponential(1.0 / args.rate, size=n_req)
     # two servers: the small class takes most of each CU, the large class the rest
+    # The small server leaves one wave slot per SIMD free (5 of its 6
+    # resident workgroups per CU) so that the large server's waves are
+    # resident beside it.  The large server's stacks hold a full path of
+    # any large request (admission rejects requests needing more).
+    large_depth = max(min(pool[gi].n, ubs[gi][0] + 1) + 1 for gi in large_ids)
     small_srv = vcgpu.Server(ctx, max_vertices=512, slots=4096, arena_bytes=256 << 20,
                              num_blocks=args.small_blocks, time_limit_ms=args.time_limit_ms)
     large_srv = vcgpu.Server(ctx, max_vertices=20000, slots=64, arena_bytes=256 << 20, block_dim=64,
-                             num_blocks=args.large_blocks, stack_depth=64, time_limit_ms=args.time_limit_ms)
+                             num_blocks=args.large_blocks, stack_depth=large_depth, time_limit_ms=args.time_limit_ms)
     arrivals = [0] * n_req
     tickets = [None] * n_req
     results = [None] * n_req
@@ -152,14 +169,13 @@ def served(ctx, args, out):
             while tickets[r] is None:
                 time.sleep(0.0002)
             srv, t = tickets[r]
-            out = srv.wait(t, args.time_limit_ms, want_cover=False)
-            results[r] = out
+            results[r] = srv.wait(t, args.time_limit_ms, want_cover=reqs[r][3] == "large")
 
     th = threading.Thread(target=collect)
     th.start()
     start = vcgpu.now_ns()
     due = start
-    for r, (gi, job, want) in enumerate(reqs):
+    for r, (gi, job, want, cls) in enumerate(reqs):
         due += int(gaps[r] * 1e9)
         while True:
             left = due - vcgpu.now_ns()
@@ -167,38 +183,53 @@ def served(ctx, args, out):
                 break
             if left > 300_000:
                 time.sleep((left - 200_000) / 1e9)
-        srv = large_srv if pool[gi].n > 512 else small_srv
+        srv = large_srv if cls == "large" else small_srv
         arrivals[r] = vcgpu.now_ns()
         tickets[r] = (srv, srv.submit(pool[gi], job))
     th.join()
     info_s = small_srv.stop()
     info_l = large_srv.stop()
-    lat, mism, unsolved, last = [], 0, 0, start
-    for r, (gi, job, want) in enumerate(reqs):
+    lat = {"small": [], "large": []}
+    exact = {"small": 0, "large": 0}
+    mism, budget_hit, last = 0, 0, start
+    for r, (gi, job, want, cls) in enumerate(reqs):
         if results[r] is None:
-            unsolved += 1
             continue
         res, done_ns = results[r]
-        lat.append((done_ns - arrivals[r]) / 1e6)
+        lat[cls].append((done_ns - arrivals[r]) / 1e6)
         last = max(last, done_ns)
-        if not res.solved:
-            unsolved += 1
+        if not res.solved or want is None:
+            budget_hit += int(not res.solved)
             continue
-        got = res.minimum if want[0] == "mvc" else res.k_found
-        mism += int(got != want[1])
+        if want[0] == "mvc":
+            ok = res.minimum == want[1]
+        elif cls == "large":
+            # k = U: "yes" only counts with a cover that checks
+            e = large_edges[gi]
+            cov = res.cover
+            ok = (res.k_found == 1 and cov is not None and int(cov.sum()) <= job.k
+                  and bool(np.all(cov[e[:, 0]] | cov[e[:, 1]])))
+        else:
+            ok = res.k_found == want[1]
+        mism += int(not ok)
+        exact[cls] += int(ok)
     wall = (last - start) / 1e9
-    lat = np.asarray(lat)
-    solved = n_req - unsolved
+    both = np.asarray(lat["small"] + lat["large"])
+    pct = lambda a, q: float(np.percentile(a, q)) if len(a) else None  # noqa: E731
+    solved = exact["small"] + exact["large"]
     out.update({
         "value": solved / wall, "unit": "graphs/s", "ms_per_step": wall * 1000.0,
         "config": {"workload": f"served stream, Poisson {args.rate:g} req/s, {n_req} requests, 95% batch-distribution "
-                               f"graphs + 5% BA n=20000 m=4, 50/50 MVC / PVC k=opt+-1",
-                   "requests": n_req, "solved": solved, "mismatches_vs_alone": mism,
-                   "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
-                   "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
-                   "mean_ms": float(lat.mean()) if len(lat) else None,
-                   "large_requests": sum(1 for gi, _, _ in reqs if pool[gi].n > 512),
+                               f"graphs (MVC / PVC k=opt+-1) + 5% BA n=20000 m=4 (MVC / PVC k=U, node budget "
+                               f"{args.large_budget})",
+                   "requests": n_req, "exact": solved, "mismatches": mism, "unsolved_at_budget": budget_hit,
+                   "small_requests": sum(1 for q in reqs if q[3] == "small"), "small_exact": exact["small"],
+                   "large_requests": sum(1 for q in reqs if q[3] == "large"), "large_exact": exact["large"],
+                   "p50_ms": pct(both, 50), "p99_ms": pct(both, 99), "mean_ms": float(both.mean()) if len(both) else None,
+                   "p50_ms_small": pct(lat["small"], 50), "p99_ms_small": pct(lat["small"], 99),
+                   "p50_ms_large": pct(lat["large"], 50), "p99_ms_large": pct(lat["large"], 99),
                    "precompute_s": precompute_s, "small_workers": info_s.workers, "large_workers": info_l.workers,
+                   "large_stack_depth": large_depth,
                    "tree_nodes": info_s.tree_nodes + info_l.tree_nodes,
                    "idle_fraction_small": info_s.idle_fraction},
         "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
@@ -215,8 +246,9 @@ def main():
                     choices=["batch", "single_er", "single_reg3", "single_ba", "served"])
     ap.add_argument("--requests", type=int, default=2000, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=1000.0, help="served: Poisson arrival rate (requests/s)")
-    ap.add_argument("--small-blocks", type=int, default=256 * 6, help="served: workgroups of the small-graph server")
-    ap.add_argument("--large-blocks", type=int, default=256, help="served: workgroups of the large-graph server")
+    ap.add_argument("--small-blocks", type=int, default=256 * 5, help="served: workgroups of the small-graph server")
+    ap.add_argument("--large-blocks", type=int, default=64, help="served: one-wave workgroups of the large-graph server")
+    ap.add_argument("--large-budget", type=int, default=1 << 16, help="served: node budget of a large request")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--base-seed", type=int, default=1000)
     ap.add_argument("--max-nodes", type=int, default=1 << 20, help="per-graph GPU tree-node budget (batch)")
@@ -224,6 +256,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner and graph capture")
     ap.add_argument("--block-dim", type=int, default=0)
     ap.add_argument("--num-blocks", type=int, default=0)
     ap.add_argument("--time-limit-ms", type=int, default=120000)
@@ -251,31 +284,56 @@ def main():
         ubs = [vcgpu.upper_bound(g, seed=args.base_seed) for g in graphs]
         jobs = [vcgpu.Job(vcgpu.VC_MVC, 0, u, args.max_nodes) for u, _ in ubs]
         gset = ctx.upload(graphs)
-        for _ in range(args.warmup):
+        # launch shape: the autotuner on a 512-graph sample of the same
+        # distribution (short node budget), then graph-captured dispatch
+        tuned = None
+        if not args.no_tune:
+            t0 = time.perf_counter()
+            ns = min(512, len(graphs))
+            sample = ctx.upload(graphs[:ns])
+            sjobs = [vcgpu.Job(vcgpu.VC_MVC, 0, u, 1 << 16) for u, _ in ubs[:ns]]
+            best, _ = ctx.autotune(sample, sjobs, params)
+            sample.free()
+            params = vcgpu.Params(version=params.version, block_dim=best.block_dim, num_blocks=args.num_blocks,
+                                  use_global_memory=best.use_global_memory, list_size=best.list_size,
+                                  list_threshold=args.list_threshold, time_limit_ms=args.time_limit_ms,
+                                  flags=vcgpu.VC_FLAG_GRAPH)
+            tuned = {"block_dim": best.block_dim, "use_global_memory": best.use_global_memory,
+                     "list_size": best.list_size, "autotune_s": time.perf_counter() - t0}
+            print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)
+        for i in range(args.warmup):
             ctx.solve(gset, jobs, params, want_covers=False, raise_on_error=False)
+            print(f"bench: warmup {i} done", file=sys.stderr, flush=True)
         walls, kms, last = [], [], None
-        for _ in range(args.steps):
+        for i in range(args.steps):
             t0 = time.perf_counter()
             res, info, st = ctx.solve(gset, jobs, params, want_covers=False, raise_on_error=False)
             walls.append(time.perf_counter() - t0)
             kms.append(info.kernel_ms)
             last = (res, info, st)
+            print(f"bench: step {i} {walls[-1]:.3f} s, kernel {info.kernel_ms:.1f} ms, "
+                  f"{sum(1 for r in res if r.solved)} solved", file=sys.stderr, flush=True)
         res, info, st = last
         solved = sum(1 for r in res if r.solved)
         nodes = sum(r.tree_nodes for r in res)
         wall = float(np.mean(walls))
         kernel_s = float(np.mean(kms)) / 1000.0
         alg_bytes = sum(r.tree_nodes * b_node(n, m) for r, (n, m, _) in zip(res, meta))
-        # plain: one graph per launch on a sample
+        # plain: one graph per launch (default launch shape) on a sample,
+        # every graph already resident in HBM before the clock starts
         ps = min(args.plain_sample, len(graphs))
+        ones = [ctx.upload(graphs[i]) for i in range(ps)]
+        plain_params = vcgpu.Params(version=vcgpu.VC_HYBRID, time_limit_ms=args.time_limit_ms)
         t0 = time.perf_counter()
         plain_solved = 0
         for i in range(ps):
-            one = ctx.upload(graphs[i])
-            (r,), _, _ = ctx.solve(one, jobs[i], params, want_covers=False, raise_on_error=False)
+            (r,), _, _ = ctx.solve(ones[i], jobs[i], plain_params, want_covers=False, raise_on_error=False)
             plain_solved += int(r.solved)
-            one.free()
         plain_wall = max(time.perf_counter() - t0, 1e-9)
+        for one in ones:
+            one.free()
+        if ps:
+            print(f"bench: plain {ps} graphs {plain_wall:.3f} s, {plain_solved} solved", file=sys.stderr, flush=True)
         out.update({
             "value": solved / wall, "unit": "graphs/s", "ms_per_step": wall * 1000.0,
             "config": {"workload": "batched MVC, 4096 graphs n~U[40,400], half ER p~U[0.02,0.15], half BA "
@@ -286,8 +344,11 @@ def main():
                        "workers": info.workers, "block_dim": info.block_dim, "num_blocks": info.num_blocks,
                        "blocks_per_cu": info.blocks_per_cu, "status": st,
                        "idle_fraction": info.idle_fraction, "donations": info.donations, "steals": info.steals,
+                       "tuned": tuned, "graph_captured": bool(params.flags & vcgpu.VC_FLAG_GRAPH),
                        "plain_graphs_per_s": plain_solved / plain_wall, "plain_sample": ps,
-                       "batched_over_plain": (solved / wall) / max(plain_solved / plain_wall, 1e-12)},
+                       "plain_solved": plain_solved,
+                       "batched_over_plain": ((solved / wall) / (plain_solved / plain_wall)
+                                              if ps and plain_solved else None)},
             "roofline": {"bound": "hbm", "achieved": alg_bytes / kernel_s / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": alg_bytes / kernel_s / 1e9 / HBM_PEAK_GBS, "traffic": None},
         })

@@ -187,7 +187,8 @@ typedef struct {
 vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **out);
 /* Queue one request (graph copied; U = job->upper_bound must be a valid
  * cover size).  *ticket identifies it in poll/wait.  Graphs larger than
- * max_vertices are rejected with VC_EINVAL. */
+ * max_vertices are rejected with VC_EINVAL; a request whose stack need
+ * min(n, max(U, k+1)) + 1 exceeds the server's stack_depth with VC_ENOTSUP. */
 vc_status vc_server_submit(vc_server *s, const vc_csr *graph, const vc_job *job, uint64_t *ticket);
 /* VC_EAGAIN while pending; otherwise fills *result (and cover, vertexNum
  * bytes, if non-NULL) and forgets the ticket.  *done_ns (optional) is the

@@ -1052,7 +1052,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                         break;
                     }
                     uint32_t late = 0;
-                    if (lane_id() == 0) late = past_deadline(a);
+                    if (lane_id() == 0) late = ld_rlx(&a.glob->cancel) || ld_rlx(&gs->stop) || past_deadline(a);
                     if (uni(late)) break;
                     backoff(spins < 4 ? spins : 4);
                 }
@@ -2231,6 +2231,13 @@ vc_status vc_server_submit(vc_server *s, const vc_csr *g, const vc_job *job, uin
     if (g->vertexNum > s->cfg.max_vertices || g->vertexNum == 0) return VC_EINVAL;
     if (job->instance != VC_MVC && job->instance != VC_PVC) return VC_EINVAL;
     if (job->upper_bound > g->vertexNum) return VC_EINVAL;
+    // Admission by stack footprint: a worker's stack must hold every entry of
+    // one root-to-leaf path of this request, min(n, max(U, k+1)) + 1 slots,
+    // so the local stacks never overflow into a full worklist.
+    {
+        const uint32_t t = job->instance == VC_PVC ? std::max(job->upper_bound, job->k + 1u) : job->upper_bound;
+        if (std::min(g->vertexNum, t) + 1u > s->pl.depth) return VC_ENOTSUP;
+    }
     if (!csr_is_valid(*g)) return VC_EBADGRAPH;
     auto rq = std::make_unique<ServerRequest>();
     rq->job = *job;

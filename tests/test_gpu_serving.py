@@ -98,3 +98,26 @@ def test_server_rejects_oversized_and_reports_pending(ctx):
         assert srv.poll(t) is None  # a ticket is forgotten once its result was taken
     finally:
         srv.stop()
+
+
+def test_server_admits_by_stack_footprint(ctx):
+    """A request whose root-to-leaf path may need more stack entries than
+    the server's stack_depth is refused at submit (its local stack could
+    otherwise overflow into a full worklist); one that fits is served."""
+    srv = vcgpu.Server(ctx, max_vertices=200, slots=4, stack_depth=40, time_limit_ms=30000)
+    try:
+        n, e = 120, graphgen.regular3_rewired(120, 3)
+        g = vcgpu.csr_from_edges(n, e)
+        U, _ = vcgpu.upper_bound(g)
+        assert min(n, U) + 1 > 40
+        with pytest.raises(vcgpu.VCError) as ei:
+            srv.submit(g, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0))
+        assert ei.value.status == vcgpu.VC_ENOTSUP
+        n2, e2 = 60, graphgen.erdos_renyi(60, 0.05, 4)
+        g2 = vcgpu.csr_from_edges(n2, e2)
+        U2, _ = vcgpu.upper_bound(g2)
+        assert min(n2, U2) + 1 <= 40
+        out = srv.wait(srv.submit(g2, vcgpu.Job(vcgpu.VC_MVC, 0, U2, 0)), 30000)
+        assert out is not None and out[0].solved and out[0].minimum == O.mvc(O.from_edges(n2, e2))[0]
+    finally:
+        srv.stop()
