@@ -282,19 +282,54 @@ __device__ void apply_list(const N &w, uint32_t L)
     w.sync();
 }
 
+// Per-lane statistics of the degree array, gathered by every rule scan in the
+// same pass that reads the degrees for the rule.  After a scan that marked
+// nothing they describe the current node exactly; that replaces the separate
+// findNumOfEdges / findMaxDegree passes and tells which rule scans cannot
+// mark anything.
+struct Lanes {
+    int s;        // sum of live degrees
+    int md;       // max degree of the lane's vertices
+    uint32_t mv;  // lowest index with degree md in the lane
+    uint32_t two; // some vertex of the lane has degree 2
+};
+
+__device__ __forceinline__ void lanes_reset(Lanes &st)
+{
+    st.s = 0;
+    st.md = 0;
+    st.mv = kNone;
+    st.two = 0u;
+}
+
+// Vertices reach a lane in increasing order, so a strict > keeps the lowest
+// index of the maximum (src/Sequential.cpp:50-58).
+__device__ __forceinline__ void lanes_add(Lanes &st, int dv, uint32_t v)
+{
+    if (dv > 0) st.s += dv;
+    if (dv > st.md) {
+        st.md = dv;
+        st.mv = v;
+    }
+    st.two |= (uint32_t)(dv == 2);
+}
+
 // Leaf rule (src/helperFunctions.cuh:229-296): for every degree-1 vertex v
 // mark its live neighbour u when deg(u) > 1, or deg(u) == 1 and u < v (so an
 // isolated edge loses exactly one end).  Repeat until nothing is marked.
 template <class N>
-__device__ uint32_t rule_leaf(const N &w)
+__device__ uint32_t rule_leaf(const N &w, Lanes &st)
 {
     uint32_t total = 0;
     for (;;) {
         uint32_t L = 0;
+        lanes_reset(st);
         for (uint32_t base = 0; base < w.n; base += kWave) {
             uint32_t v = base + lane_id();
             uint32_t cand = kNone;
-            if (v < w.n && w.deg[v] == 1) {
+            int dv = v < w.n ? (int)w.deg[v] : -1;
+            lanes_add(st, dv, v);
+            if (dv == 1) {
                 uint32_t e = w.src[v + 1];
                 for (uint32_t x = w.src[v]; x < e; ++x) {
                     uint32_t u = w.dst[x];
@@ -335,15 +370,18 @@ __device__ __forceinline__ bool adj_has(const uint32_t *dst, uint32_t lo, uint32
 // cover.  Skipped when a neighbour has degree 1, or degree 2 with a lower
 // index, as in the reference.  All triangles of a pass are applied together.
 template <class N>
-__device__ uint32_t rule_triangle(const N &w)
+__device__ uint32_t rule_triangle(const N &w, Lanes &st)
 {
     uint32_t total = 0;
     for (;;) {
         uint32_t L = 0;
+        lanes_reset(st);
         for (uint32_t base = 0; base < w.n; base += kWave) {
             uint32_t v = base + lane_id();
             uint32_t c1 = kNone, c2 = kNone;
-            if (v < w.n && w.deg[v] == 2) {
+            int dv = v < w.n ? (int)w.deg[v] : -1;
+            lanes_add(st, dv, v);
+            if (dv == 2) {
                 uint32_t n1 = kNone, n2 = kNone;
                 bool keep = false;
                 uint32_t e = w.src[v + 1];
@@ -381,16 +419,18 @@ __device__ uint32_t rule_triangle(const N &w)
 // High-degree rule (src/helperFunctions.cuh:299-358): v goes into the cover
 // when deg(v) + d + (deleted so far in this call) >= T.
 template <class N>
-__device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T)
+__device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st)
 {
     uint32_t total = 0;
     for (;;) {
         uint32_t L = 0;
+        lanes_reset(st);
         for (uint32_t base = 0; base < w.n; base += kWave) {
             uint32_t v = base + lane_id();
             uint32_t cand = kNone;
             if (v < w.n) {
                 int dv = w.deg[v];
+                lanes_add(st, dv, v);
                 if (dv > 0 && (uint32_t)dv + d + total >= T) cand = v;
             }
             L = append(w, L, cand, false);
@@ -402,60 +442,50 @@ __device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T)
     return total;
 }
 
-// Rule fixpoint in the order of src/GlobalWorkList.cuh:107-130.  Returns the
-// new number of vertices in the cover.
-template <class N>
-__device__ uint32_t reduce_node(const N &w, uint32_t d, uint32_t T)
-{
-    uint32_t it = 0, nl, nt = 0, nh = 0;
-    do {
-        nl = rule_leaf(w);
-        d += nl;
-        if (it == 0 || nl > 0 || nh > 0) {
-            nt = rule_triangle(w);
-            d += nt;
-        } else {
-            nt = 0;
-        }
-        if (it == 0 || nl > 0 || nt > 0) {
-            nh = rule_high(w, d, T);
-            d += nh;
-        } else {
-            nh = 0;
-        }
-        ++it;
-    } while (nt > 0 || nh > 0);
-    return d;
-}
-
 struct Summary {
     uint32_t edges;  // live edges (findNumOfEdges, src/helperFunctions.cuh:483-504)
     int32_t maxdeg;  // findMaxDegree (src/helperFunctions.cuh:450-481)
     uint32_t maxv;   // lowest index of maximum degree
 };
 
+// Rule fixpoint in the order of src/GlobalWorkList.cuh:107-130.  Returns the
+// new number of vertices in the cover and, in *sum, the node's edge count and
+// max-degree vertex.  Every rule call ends with a scan that marked nothing,
+// so the statistics of the latest scan are exact for the current degrees:
+//  * the triangle rule is not scanned when no vertex has degree 2 (it could
+//    not mark anything);
+//  * the high-degree rule is not scanned when max degree + d < T (its first
+//    scan could not mark anything);
+//  * the summary needs no scan of its own.
+// The rules that run, their order and their results are unchanged.
 template <class N>
-__device__ Summary summarize(const N &w)
+__device__ uint32_t reduce_node(const N &w, uint32_t d, uint32_t T, Summary *sum)
 {
-    int s = 0, md = 0;
-    uint32_t mv = kNone;
-    for (uint32_t base = 0; base < w.n; base += kWave) {
-        uint32_t v = base + lane_id();
-        if (v < w.n) {
-            int dv = w.deg[v];
-            if (dv > 0) s += dv;
-            if (dv > md) {
-                md = dv;
-                mv = v;
+    uint32_t it = 0, nl, nt = 0, nh = 0;
+    Lanes st;
+    do {
+        nl = rule_leaf(w, st);
+        d += nl;
+        nt = 0;
+        if ((it == 0 || nl > 0 || nh > 0) && __ballot(st.two != 0u) != 0ull) {
+            nt = rule_triangle(w, st);
+            d += nt;
+        }
+        nh = 0;
+        if (it == 0 || nl > 0 || nt > 0) {
+            const int md = wave_max(st.md);
+            if ((uint32_t)md + d >= T) {
+                nh = rule_high(w, d, T, st);
+                d += nh;
             }
         }
-    }
-    Summary r;
-    r.edges = (uint32_t)wave_sum(s) / 2u;
-    r.maxdeg = wave_max(md);
-    uint32_t cand = (md == r.maxdeg && r.maxdeg > 0) ? mv : kNone;
-    r.maxv = wave_min_u32(cand);
-    return r;
+        ++it;
+    } while (nt > 0 || nh > 0);
+    const int md = wave_max(st.md);
+    sum->edges = (uint32_t)wave_sum(st.s) / 2u;
+    sum->maxdeg = md;
+    sum->maxv = wave_min_u32((st.md == md && md > 0) ? st.mv : kNone);
+    return d;
 }
 
 // Left child in place: v into the cover (deleteMaxDegreeVertex,
@@ -999,8 +1029,8 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
 
         const bool mvc = gd.instance == 0;
         uint32_t T = mvc ? (uint32_t)uni((uint32_t)ld_rlx(&gs->best)) : gd.k + 1u;
-        d = reduce_node(w, d, T);
-        Summary s = summarize(w);
+        Summary s;
+        d = reduce_node(w, d, T, &s);
 
         bool prune;
         if (mvc) {
