@@ -37,6 +37,105 @@ import graphgen  # noqa: E402
 import vcgpu  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+KERNEL_SOURCES = [os.path.join(PKG, "csrc", f) for f in ("vc_kernels.hip", "vc_device.hpp")]
+BATCH_PROFILE = os.path.join(ROOT, "profiles", "solve_kernel_batch4096.md")
+
+
+def kernel_source_sha() -> str:
+    """Hash of the kernel sources a committed profile was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def committed_traffic(path: str = BATCH_PROFILE):
+    """HBM bytes per launch of the solve kernel on the batched workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary; None when there
+    is none or it was measured on other kernel sources."""
+    import re
+    try:
+        with open(path) as fh:
+            text = fh.read()
+    except OSError:
+        return None, None
+    kv = dict(re.findall(r"^- (\w+): `([^`]*)`", text, re.M))
+    if kv.get("kernel_source_sha256") != kernel_source_sha() or "traffic_bytes_per_launch" not in kv:
+        return None, "stale"
+    return float(kv["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+
+
+def _rocpd(db, sql):
+    import sqlite3
+    con = sqlite3.connect(db)
+    try:
+        return con.execute(sql).fetchall()
+    finally:
+        con.close()
+
+
+def _dispatches(db):
+    """(kernel, dispatch id, duration ns) in start order."""
+    rows = _rocpd(db, "select s.kernel_name, d.id, d.end - d.start from rocpd_kernel_dispatch d "
+                      "join rocpd_info_kernel_symbol s on s.id = d.kernel_id order by d.start")
+    return [(k.split("(")[0], i, dur) for k, i, dur in rows]
+
+
+def _pmc(db):
+    """{(dispatch id, counter): value} summed over the counter's instances."""
+    rows = _rocpd(db, "select d.id, p.name, e.value from rocpd_pmc_event e join rocpd_info_pmc p on p.id = e.pmc_id "
+                      "join rocpd_kernel_dispatch d on d.event_id = e.event_id")
+    out = {}
+    for i, name, v in rows:
+        out[(i, name)] = out.get((i, name), 0.0) + v
+    return out
+
+
+def rocpd_summary(dbs, long_ms: float = 100.0):
+    """Markdown lines for rocprofv3 result databases: per-kernel dispatch
+    counts and mean durations, every dispatch longer than long_ms, and the
+    counters of those dispatches."""
+    lines = []
+    for db in dbs:
+        lines.append(f"### `{os.path.basename(db)}`")
+        ds = _dispatches(db)
+        pmc = _pmc(db)
+        per = {}
+        for k, i, dur in ds:
+            per.setdefault(k, []).append((i, dur))
+        for k, v in per.items():
+            durs = [d for _, d in v]
+            lines.append(f"- {k}: {len(v)} dispatches, mean {sum(durs) / len(durs) / 1e6:.3f} ms")
+            for i, d in v:
+                if d / 1e6 >= long_ms or "calib" in k:
+                    ctr = ", ".join(f"{n} = {val:.6g}" for (j, n), val in sorted(pmc.items()) if j == i)
+                    lines.append(f"  - dispatch {i}: {d / 1e6:.3f} ms" + (f"; {ctr}" if ctr else ""))
+    return lines
+
+
+def traffic_from(fetch_db, write_db, cal_fetch_db, cal_write_db, cal_bytes: int, kernel: str = "solve_kernel"):
+    """HBM bytes of the last long dispatch of `kernel` (the timed launch):
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the factor
+    that makes the same counter read cal_bytes for the calibration copy
+    kernel (which reads and writes cal_bytes per dispatch)."""
+    def last(db, counter):
+        pmc = _pmc(db)
+        ds = [(i, dur) for k, i, dur in _dispatches(db) if kernel in k]
+        i, dur = max(ds, key=lambda x: x[0]) if ds else (None, 0)
+        return pmc.get((i, counter)), dur
+    def calib(db, counter):
+        pmc = _pmc(db)
+        vals = [pmc.get((i, counter)) for k, i, _ in _dispatches(db) if "calib_copy" in k]
+        vals = [v for v in vals if v]
+        return cal_bytes / (sum(vals) / len(vals) * 1024.0)
+    f_kib, f_dur = last(fetch_db, "FETCH_SIZE")
+    w_kib, w_dur = last(write_db, "WRITE_SIZE")
+    ff, wf = calib(cal_fetch_db, "FETCH_SIZE"), calib(cal_write_db, "WRITE_SIZE")
+    total = f_kib * 1024.0 * ff + w_kib * 1024.0 * wf
+    return {"fetch_kib": f_kib, "write_kib": w_kib, "fetch_factor": ff, "write_factor": wf,
+            "fetch_dispatch_ms": f_dur / 1e6, "write_dispatch_ms": w_dur / 1e6, "traffic_bytes": total}
 
 
 def b_node(n: int, m: int) -> int:
@@ -262,7 +361,22 @@ def main():
     ap.add_argument("--time-limit-ms", type=int, default=120000)
     ap.add_argument("--list-size", type=int, default=0, help="worklist slots (-q), 0 = auto")
     ap.add_argument("--list-threshold", type=float, default=0.0, help="donation threshold fraction (-t), 0 = auto")
+    ap.add_argument("--rocpd-summary", nargs="+", metavar="DB",
+                    help="print a markdown summary of rocprofv3 result databases and exit (no GPU use)")
+    ap.add_argument("--traffic-from", nargs=4, metavar=("FETCH_DB", "WRITE_DB", "CAL_FETCH_DB", "CAL_WRITE_DB"),
+                    help="print the calibrated HBM traffic of the timed solve launch and exit (no GPU use)")
+    ap.add_argument("--calib-bytes", type=int, default=1 << 28, help="bytes the calibration copy reads/writes")
     args = ap.parse_args()
+    if args.rocpd_summary or args.traffic_from:
+        if args.rocpd_summary:
+            print("\n".join(rocpd_summary(args.rocpd_summary)))
+        if args.traffic_from:
+            t = traffic_from(*args.traffic_from, cal_bytes=args.calib_bytes)
+            for k, v in t.items():
+                print(f"- {k}: `{v:.6g}`")
+            print(f"- traffic_bytes_per_launch: `{t['traffic_bytes']:.6g}`")
+            print(f"- kernel_source_sha256: `{kernel_source_sha()}`")
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -319,6 +433,8 @@ def main():
         wall = float(np.mean(walls))
         kernel_s = float(np.mean(kms)) / 1000.0
         alg_bytes = sum(r.tree_nodes * b_node(n, m) for r, (n, m, _) in zip(res, meta))
+        traffic, traffic_src = (committed_traffic() if args.count == 4096 and args.max_nodes == 1 << 20
+                                else (None, None))
         # plain: one graph per launch (default launch shape) on a sample,
         # every graph already resident in HBM before the clock starts
         ps = min(args.plain_sample, len(graphs))
@@ -350,7 +466,9 @@ def main():
                        "batched_over_plain": ((solved / wall) / (plain_solved / plain_wall)
                                               if ps and plain_solved else None)},
             "roofline": {"bound": "hbm", "achieved": alg_bytes / kernel_s / 1e9, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": alg_bytes / kernel_s / 1e9 / HBM_PEAK_GBS, "traffic": None},
+                         "unit": "GB/s", "frac": alg_bytes / kernel_s / 1e9 / HBM_PEAK_GBS,
+                         "traffic": traffic, "traffic_source": traffic_src,
+                         "algorithmic_bytes": alg_bytes, "kernel_ms": kernel_s * 1000.0},
         })
         if not args.no_cpu:
             th = args.cpu_threads or min(16, os.cpu_count() or 1)

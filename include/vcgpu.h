@@ -83,6 +83,10 @@ typedef struct {
  * solve kernel and result copies captured once per graph set and launch
  * shape, replayed on later calls with the same shape). */
 #define VC_FLAG_GRAPH 1u
+/* vc_params.flags: store degrees as int32 (with 64-bit sketch sums) even
+ * when every graph has fewer than 32768 vertices, as graphs beyond that size
+ * always do.  Only useful to exercise that storage variant on small inputs. */
+#define VC_FLAG_WIDE 2u
 
 /* Per-graph outcome.  Replaces the 4-byte minimum_d / kFound_d readback of
  * src/main.cu:355-359 and adds the node count and status. */

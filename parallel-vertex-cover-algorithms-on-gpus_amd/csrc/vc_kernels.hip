@@ -177,17 +177,43 @@ __device__ __forceinline__ bool past_deadline(const SolveArgs &a)
 // one worker's view of its current tree node
 // ---------------------------------------------------------------------------
 
+// Neighbour sketch of a tree node.  Beside its degree (number of live
+// neighbours; -1 = in the cover) every vertex v keeps
+//   s1[v] = sum of the ids of its live neighbours      (mod 2^32)
+//   s2[v] = sum of the squared ids of its live neighbours (mod 2^32 or 2^64)
+// so a degree-1 vertex's neighbour is s1[v], and a degree-2 vertex's two
+// neighbours n1 < n2 follow from n1 + n2 = s1 and n1^2 + n2^2 = s2:
+// (n2 - n1)^2 = 2 s2 - s1^2.  That replaces the reference's walk over the
+// whole original adjacency list to find the live neighbours
+// (src/helperFunctions.cuh:240-252, 372-392) with two local reads.  With
+// int16 degrees (n < 2^15) the decoded values are below 2^31, so 32-bit
+// sums suffice; wider graphs keep a 64-bit square sum.
+template <typename DegT> struct SketchOf {
+    using s2_t = uint32_t;
+};
+template <> struct SketchOf<int32_t> {
+    using s2_t = unsigned long long;
+};
+// bytes of one node (degrees + sketch) per vertex
+template <typename DegT> constexpr uint32_t node_bytes_per_vertex()
+{
+    return (uint32_t)(sizeof(DegT) + sizeof(uint32_t) + sizeof(typename SketchOf<DegT>::s2_t));
+}
+
 template <typename DegT, bool HBM>
 struct Node {
+    using deg_t = DegT;
+    using s2_t = typename SketchOf<DegT>::s2_t;
     DegT *deg;           // [npad]: degree, -1 = in cover
+    uint32_t *s1;        // [npad]: sum of live neighbour ids
+    s2_t *s2;            // [npad]: sum of squared live neighbour ids
     uint32_t *mark;      // [mwords] bitset, all zero between passes
     uint32_t *list;      // [lcap]: vertices marked in the current pass
-    int *xs;             // [2 * 64] scratch of apply_list: edge offsets, list bases
+    int *xs;             // [3 * 64] scratch of apply_list: edge offsets, list bases, list vertices
     const uint32_t *src; // graph srcPtr (HBM)
     const uint32_t *dst; // graph dst (HBM)
     uint32_t n;
     uint32_t lcap;
-    using deg_t = DegT;
 
     __device__ __forceinline__ void sync() const { nsync<HBM>(); }
 };
@@ -235,6 +261,41 @@ __device__ __forceinline__ void dec_degree(const N &w, uint32_t t)
     }
 }
 
+// Live vertex t loses its live neighbour u (several lanes may hit the same t).
+template <class N>
+__device__ __forceinline__ void drop_neighbour(const N &w, uint32_t t, uint32_t u)
+{
+    using S2 = typename N::s2_t;
+    dec_degree(w, t);
+    __hip_atomic_fetch_add(&w.s1[t], 0u - u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __hip_atomic_fetch_add(&w.s2[t], (S2)0 - (S2)u * (S2)u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// Exact integer square root of a perfect square below 2^30 (32-bit sums) or
+// 2^62 (64-bit sums); the caller checks r * r == x.
+__device__ __forceinline__ uint32_t isqrt_round(uint32_t x) { return (uint32_t)(__builtin_sqrtf((float)x) + 0.5f); }
+__device__ __forceinline__ unsigned long long isqrt_round(unsigned long long x)
+{
+    return (unsigned long long)(__builtin_sqrt((double)x) + 0.5);
+}
+
+// The two live neighbours n1 < n2 of a degree-2 vertex from its sketch.
+// False when the sketch does not decode to two distinct vertices below n
+// (impossible for a consistent node).
+template <class N>
+__device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, uint32_t &n1, uint32_t &n2)
+{
+    using S2 = typename N::s2_t;
+    const uint32_t s = w.s1[v];
+    const S2 q = w.s2[v];
+    const S2 x = (S2)2 * q - (S2)s * (S2)s;
+    const S2 r = isqrt_round(x);
+    if (r * r != x || r == 0 || r > (S2)s) return false;
+    n1 = (uint32_t)(((S2)s - r) >> 1);
+    n2 = (uint32_t)(((S2)s + r) >> 1);
+    return n2 < w.n;
+}
+
 // Apply phase of a rule: every listed vertex goes into the cover.  First all
 // of them get -1, then each decrements its neighbours that are still live
 // (src/helperFunctions.cuh:270-289).  Listed vertices are distinct and live.
@@ -249,19 +310,21 @@ __device__ void apply_list(const N &w, uint32_t L)
     for (uint32_t i = lane_id(); i < L; i += kWave) w.deg[w.list[i]] = -1;
     w.sync();
     const int lane = (int)lane_id();
-    int *const xoff = w.xs;          // exclusive edge offset of list entry j
-    int *const xbase = w.xs + kWave; // first adjacency index of list entry j
+    int *const xoff = w.xs;              // exclusive edge offset of list entry j
+    int *const xbase = w.xs + kWave;     // first adjacency index of list entry j
+    int *const xvert = w.xs + 2 * kWave; // list entry j
     for (uint32_t c0 = 0; c0 < L; c0 += kWave) {
         const uint32_t cnt = min(L - c0, (uint32_t)kWave);
-        int b = 0, len = 0;
+        int b = 0, len = 0, u = 0;
         if ((uint32_t)lane < cnt) {
-            const uint32_t u = w.list[c0 + lane];
+            u = (int)w.list[c0 + lane];
             b = (int)w.src[u];
             len = (int)w.src[u + 1] - b;
         }
         const int incl = wave_incl_scan(len);
         xoff[lane] = incl - len;
         xbase[lane] = b;
+        xvert[lane] = u;
         const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
         w.sync();
 #pragma unroll 1
@@ -274,7 +337,7 @@ __device__ void apply_list(const N &w, uint32_t L)
                 if (cand < (int)cnt && xoff[cand] <= x) pos = cand;
             }
             const uint32_t t = w.dst[xbase[pos] + (x - xoff[pos])];
-            if (w.deg[t] != -1) dec_degree(w, t);
+            if (w.deg[t] != -1) drop_neighbour(w, t, (uint32_t)xvert[pos]);
         }
         w.sync();
     }
@@ -330,14 +393,10 @@ __device__ uint32_t rule_leaf(const N &w, Lanes &st)
             int dv = v < w.n ? (int)w.deg[v] : -1;
             lanes_add(st, dv, v);
             if (dv == 1) {
-                uint32_t e = w.src[v + 1];
-                for (uint32_t x = w.src[v]; x < e; ++x) {
-                    uint32_t u = w.dst[x];
-                    int du = w.deg[u];
-                    if (du > 0) {
-                        if (du > 1 || u < v) cand = u;
-                        break;
-                    }
+                const uint32_t u = w.s1[v];
+                if (u < w.n) {
+                    const int du = w.deg[u];
+                    if (du > 1 || (du == 1 && u < v)) cand = u;
                 }
             }
             L = append(w, L, cand, true);
@@ -381,27 +440,13 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st)
             uint32_t c1 = kNone, c2 = kNone;
             int dv = v < w.n ? (int)w.deg[v] : -1;
             lanes_add(st, dv, v);
-            if (dv == 2) {
-                uint32_t n1 = kNone, n2 = kNone;
-                bool keep = false;
-                uint32_t e = w.src[v + 1];
-                for (uint32_t x = w.src[v]; x < e; ++x) {
-                    uint32_t u = w.dst[x];
-                    int du = w.deg[u];
-                    if (du > 0) {
-                        if (du == 1 || (du == 2 && u < v)) {
-                            keep = true;
-                            break;
-                        }
-                        if (n1 == kNone) {
-                            n1 = u;
-                        } else {
-                            n2 = u;
-                            break;
-                        }
-                    }
-                }
-                if (!keep && n2 != kNone && adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
+            uint32_t n1, n2;
+            if (dv == 2 && two_neighbours(w, v, n1, n2)) {
+                // skipped when a neighbour has degree 1, or degree 2 and a
+                // lower index (src/helperFunctions.cuh:379-385)
+                const int d1 = w.deg[n1], d2 = w.deg[n2];
+                const bool skip = d1 == 1 || (d1 == 2 && n1 < v) || d2 == 1 || (d2 == 2 && n2 < v);
+                if (!skip && adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
                     c1 = n1;
                     c2 = n2;
                 }
@@ -496,11 +541,17 @@ __device__ void take_vertex(const N &w, uint32_t v)
     w.sync();
     if (lane_id() == 0) w.deg[v] = -1;
     w.sync();
+    using S2 = typename N::s2_t;
     uint32_t e = w.src[v + 1];
     for (uint32_t x = w.src[v] + lane_id(); x < e; x += kWave) {
         uint32_t t = w.dst[x];
         int dt = w.deg[t];
-        if (dt != -1) w.deg[t] = (typename N::deg_t)(dt - 1);
+        if (dt != -1) {
+            // the neighbours of one vertex are distinct: plain updates
+            w.deg[t] = (typename N::deg_t)(dt - 1);
+            w.s1[t] -= v;
+            w.s2[t] -= (S2)v * (S2)v;
+        }
     }
     w.sync();
 }
@@ -530,6 +581,32 @@ __device__ uint32_t take_neighbours(const N &w, uint32_t v)
         total += L;
     }
     return total;
+}
+
+// Root node of a graph: its degrees, and the sketch sums over every
+// neighbour (all are live).  Padding vertices [n, npad) are in the cover.
+template <class N>
+__device__ void init_root(const N &w, const int32_t *deg0, uint32_t npad)
+{
+    using S2 = typename N::s2_t;
+    for (uint32_t v = lane_id(); v < npad; v += kWave) {
+        uint32_t s = 0;
+        S2 q = 0;
+        if (v < w.n) {
+            w.deg[v] = (typename N::deg_t)deg0[v];
+            const uint32_t e = w.src[v + 1];
+            for (uint32_t x = w.src[v]; x < e; ++x) {
+                const uint32_t u = w.dst[x];
+                s += u;
+                q += (S2)u * (S2)u;
+            }
+        } else {
+            w.deg[v] = (typename N::deg_t)-1;
+        }
+        w.s1[v] = s;
+        w.s2[v] = q;
+    }
+    w.sync();
 }
 
 // Node storage <-> HBM slot copies (bytes % 16 == 0, both sides 16-aligned).
@@ -571,10 +648,13 @@ __device__ bool try_enqueue(const SolveArgs &a, const void *src, uint32_t g, uin
             while (ld_acq(&a.tickets[slot]) != ticket) {
                 backoff(spins < 3 ? spins : 3);
                 ++spins;
-                if (past_deadline(a)) break;
+                if (past_deadline(a)) {
+                    ok = 0;  // the launch is being cancelled: leave the slot alone
+                    break;
+                }
             }
             // the subtree being donated is one more outstanding unit of g
-            fetch_add(&a.gstate[g].outstanding, 1u);
+            if (ok) fetch_add(&a.gstate[g].outstanding, 1u);
         } else {
             fetch_add(&a.glob->count, -1);
         }
@@ -611,7 +691,10 @@ __device__ bool try_dequeue(const SolveArgs &a, const N &w, uint32_t &g, uint32_
             while (ld_acq(&a.tickets[slot]) != ticket) {
                 backoff(spins < 3 ? spins : 3);
                 ++spins;
-                if (past_deadline(a)) break;
+                if (past_deadline(a)) {
+                    ok = 0;  // the launch is being cancelled: leave the slot alone
+                    break;
+                }
             }
         } else {
             fetch_add(&a.glob->count, 1);
@@ -917,12 +1000,15 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
     const uint32_t wid = blockIdx.x * (blockDim.x / kWave) + wib;
     if (wid >= a.workers) return;
 
-    // node storage: deg [npad DegT] | list [lcap u32] | mark [mwords u32] | xs [128 i32]
+    // node storage: deg [npad DegT] | s1 [npad u32] | s2 [npad] | list [lcap u32] | mark [mwords u32] |
+    // xs [192 i32]; a stack or worklist slot holds deg | s1 | s2
     char *store = HBM ? a.hbm_nodes + (size_t)wid * a.worker_bytes
                       : reinterpret_cast<char *>(smem4) + (size_t)wib * a.worker_bytes;
-    const uint32_t slot_bytes = a.npad * (uint32_t)sizeof(DegT);
+    const uint32_t slot_bytes = a.npad * node_bytes_per_vertex<DegT>();
     Node<DegT, HBM> w;
     w.deg = reinterpret_cast<DegT *>(store);
+    w.s1 = reinterpret_cast<uint32_t *>(store + (size_t)a.npad * sizeof(DegT));
+    w.s2 = reinterpret_cast<typename Node<DegT, HBM>::s2_t *>(store + (size_t)a.npad * (sizeof(DegT) + 4u));
     w.list = reinterpret_cast<uint32_t *>(store + slot_bytes);
     w.mark = w.list + a.lcap;
     w.xs = reinterpret_cast<int *>(w.mark + a.mwords);
@@ -990,10 +1076,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                     if (q.v != kNone) d += take_neighbours(w, q.v);
                     descend = 0;
                 } else {
-                    const int32_t *deg0 = reinterpret_cast<const int32_t *>(a.gbuf + gd.off_deg);
-                    for (uint32_t i = lane_id(); i < a.npad; i += kWave)
-                        w.deg[i] = (DegT)(i < gd.n ? deg0[i] : -1);
-                    w.sync();
+                    init_root(w, reinterpret_cast<const int32_t *>(a.gbuf + gd.off_deg), a.npad);
                     d = 0;
                     if (a.version == 1) {
                         descend = 0;
@@ -1088,7 +1171,10 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                 }
             }
             if (!moved) {
-                if (lane_id() == 0) st_rlx(&a.glob->error, kErrStackOverflow);
+                // hybrid: the wait ended because the graph stopped or the
+                // launch is cancelled; stack-only: the depth is exact, so
+                // this is an internal error
+                if (a.version != 1 && lane_id() == 0) st_rlx(&a.glob->error, kErrStackOverflow);
                 have = false;
                 continue;
             }
@@ -1152,10 +1238,10 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
 }
 
 template __global__ void solve_kernel<int16_t, false, false>(SolveArgs);
-template __global__ void solve_kernel<int32_t, false, false>(SolveArgs);
+template __global__ void solve_kernel<int16_t, true, false>(SolveArgs);
 template __global__ void solve_kernel<int32_t, true, false>(SolveArgs);
 template __global__ void solve_kernel<int16_t, false, true>(SolveArgs);
-template __global__ void solve_kernel<int32_t, false, true>(SolveArgs);
+template __global__ void solve_kernel<int16_t, true, true>(SolveArgs);
 template __global__ void solve_kernel<int32_t, true, true>(SolveArgs);
 
 }  // namespace vcg
@@ -1262,41 +1348,48 @@ bool csr_is_valid(const vc_csr &g)
     return true;
 }
 
-// Kernel variants: degree type x node storage.
-enum Variant { kLds16 = 0, kLds32 = 1, kHbm32 = 2 };
+// Kernel variants: degree type x node storage.  int16 degrees (32-bit
+// sketch sums) while every graph has n < 32768, else int32 (64-bit square
+// sums); node storage in LDS when a worker fits, else in HBM.  int32 nodes
+// need at least 512 KB at n = 32768, so they are never LDS-resident.
+enum Variant { kLds16 = 0, kHbm16 = 1, kHbm32 = 2 };
 
 const void *kernel_of(int variant, bool serve)
 {
     if (serve) {
         switch (variant) {
         case kLds16: return reinterpret_cast<const void *>(vcg::solve_kernel<int16_t, false, true>);
-        case kLds32: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, false, true>);
+        case kHbm16: return reinterpret_cast<const void *>(vcg::solve_kernel<int16_t, true, true>);
         default: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, true, true>);
         }
     }
     switch (variant) {
     case kLds16: return reinterpret_cast<const void *>(vcg::solve_kernel<int16_t, false, false>);
-    case kLds32: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, false, false>);
+    case kHbm16: return reinterpret_cast<const void *>(vcg::solve_kernel<int16_t, true, false>);
     default: return reinterpret_cast<const void *>(vcg::solve_kernel<int32_t, true, false>);
     }
 }
 
 struct Plan {
     int variant;
-    uint32_t npad, lcap, mwords, worker_bytes, deg_bytes, wpb, block_dim, blocks_per_cu, num_blocks, workers;
+    uint32_t npad, lcap, mwords, worker_bytes, deg_bytes, slot_bytes, wpb, block_dim, blocks_per_cu, num_blocks,
+        workers;
     uint32_t depth, qsize, qthreshold, roots, units;
     size_t smem;
     size_t off_glob, off_stk, off_meta, off_qdeg, off_qmeta, off_tickets, off_wnodes, off_hbm, total;
 };
 
-// Per-worker node storage for a degree type: deg [npad] | list [lcap] | mark [mwords] | xs [128].
+// Per-worker node storage for a degree type: deg [npad] | s1 [npad] | s2 [npad] | list [lcap] |
+// mark [mwords] | xs [192].  A stack / worklist slot is deg | s1 | s2.
 void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes)
 {
     pl.deg_bytes = deg_bytes;
     pl.npad = round_up(nmax, 64u / deg_bytes);
+    pl.slot_bytes = pl.npad * (deg_bytes == 2u ? vcg::node_bytes_per_vertex<int16_t>()
+                                               : vcg::node_bytes_per_vertex<int32_t>());
     pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), 1024u);
     pl.mwords = round_up((nmax + 31u) / 32u, 4u);
-    pl.worker_bytes = round_up(pl.npad * deg_bytes + 4u * pl.lcap + 4u * pl.mwords + 4u * 2u * vcg::kWave, 64u);
+    pl.worker_bytes = round_up(pl.slot_bytes + 4u * pl.lcap + 4u * pl.mwords + 4u * 3u * vcg::kWave, 64u);
 }
 
 // Launch shape shared by the batch solve and the server: node storage,
@@ -1312,16 +1405,14 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
     const uint32_t nmax = std::max<uint32_t>(nmax_in, 1u);
     const size_t lds_limit = ctx->lds_per_cu ? ctx->lds_per_cu : 160u * 1024u;
 
-    // Degree arrays in LDS when at least one worker fits per CU (int16 while
-    // every degree fits), otherwise in HBM (-g 1 forces HBM, -g 0 forbids it).
-    pl.variant = nmax < 32768u ? kLds16 : kLds32;
-    size_storage(pl, nmax, pl.variant == kLds16 ? 2u : 4u);
-    bool hbm = p->use_global_memory == 1 || pl.worker_bytes > lds_limit;
+    // Node storage in LDS when at least one worker fits per CU, otherwise in
+    // HBM (-g 1 forces HBM, -g 0 forbids it); int16 degrees while n < 32768
+    // (VC_FLAG_WIDE forces int32).
+    const bool wide = nmax >= 32768u || (p->flags & VC_FLAG_WIDE) != 0u;
+    size_storage(pl, nmax, wide ? 4u : 2u);
+    bool hbm = wide || p->use_global_memory == 1 || pl.worker_bytes > lds_limit;
     if (hbm && p->use_global_memory == 0) return VC_ENOTSUP;
-    if (hbm) {
-        pl.variant = kHbm32;
-        size_storage(pl, nmax, 4u);
-    }
+    pl.variant = wide ? kHbm32 : hbm ? kHbm16 : kLds16;
     // a worker is one wave: shrink the workgroup when LDS would not hold it
     uint32_t wpb = block_dim / vcg::kWave;
     if (!hbm && !p->block_dim)
@@ -1357,7 +1448,7 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
 
     // Fit stacks + worklist into device memory; shrink the grid if needed
     // (the reference halves the worklist instead, src/auxFunctions.cpp:507-512).
-    const size_t slot = (size_t)pl.npad * pl.deg_bytes;
+    const size_t slot = pl.slot_bytes;
     size_t budget = ctx->total_mem / 10 * 7;
     for (;;) {
         pl.workers = pl.num_blocks * pl.wpb;
@@ -1374,7 +1465,7 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
         pl.off_qmeta = take((size_t)pl.qsize * sizeof(uint4));
         pl.off_tickets = take((size_t)pl.qsize * sizeof(uint32_t));
         pl.off_wnodes = take((size_t)pl.workers * sizeof(vcg::WorkerStats));
-        pl.off_hbm = take(pl.variant == kHbm32 ? (size_t)pl.workers * pl.worker_bytes : 0);
+        pl.off_hbm = take(pl.variant != kLds16 ? (size_t)pl.workers * pl.worker_bytes : 0);
         pl.total = off;
         if (pl.total <= budget || pl.num_blocks <= 1) break;
         pl.num_blocks = std::max<uint32_t>(1u, pl.num_blocks / 2u);
@@ -1425,7 +1516,7 @@ void fill_args(SolveArgs &a, const Plan &pl, char *base, uint32_t version, uint3
     a.start_depth = version == VC_STACK_ONLY ? start_depth : 0u;
     a.roots = pl.roots;
     a.limit_ticks = limit_ticks;
-    a.hbm_nodes = pl.variant == kHbm32 ? base + pl.off_hbm : nullptr;
+    a.hbm_nodes = pl.variant != kLds16 ? base + pl.off_hbm : nullptr;
 }
 
 template <bool SERVE>
@@ -1436,9 +1527,9 @@ void launch_variant(const Plan &pl, hipStream_t stream, const SolveArgs &a)
         hipLaunchKernelGGL((vcg::solve_kernel<int16_t, false, SERVE>), dim3(pl.num_blocks), dim3(pl.block_dim),
                            pl.smem, stream, a);
         break;
-    case kLds32:
-        hipLaunchKernelGGL((vcg::solve_kernel<int32_t, false, SERVE>), dim3(pl.num_blocks), dim3(pl.block_dim),
-                           pl.smem, stream, a);
+    case kHbm16:
+        hipLaunchKernelGGL((vcg::solve_kernel<int16_t, true, SERVE>), dim3(pl.num_blocks), dim3(pl.block_dim), 0,
+                           stream, a);
         break;
     default:
         hipLaunchKernelGGL((vcg::solve_kernel<int32_t, true, SERVE>), dim3(pl.num_blocks), dim3(pl.block_dim), 0,
@@ -1507,11 +1598,11 @@ vc_status vc_ctx_create(int device, vc_ctx **out)
 vc_status vc_ctx_destroy(vc_ctx *c)
 {
     if (!c) return VC_EINVAL;
-    hipSetDevice(c->device);
-    if (c->scratch) hipFree(c->scratch);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return VC_OK;
 }
@@ -1520,7 +1611,7 @@ vc_status vc_graphs_upload(vc_ctx *ctx, int count, const vc_csr *graphs, vc_grap
 {
     if (!ctx || !out || count < 1 || !graphs) return VC_EINVAL;
     *out = nullptr;
-    hipSetDevice(ctx->device);
+    if (!hip_ok(hipSetDevice(ctx->device))) return VC_EDEVICE;
     for (int i = 0; i < count; ++i)
         if (!csr_is_valid(graphs[i])) return VC_EBADGRAPH;
 
@@ -1586,17 +1677,17 @@ vc_status vc_graphs_upload(vc_ctx *ctx, int count, const vc_csr *graphs, vc_grap
 vc_status vc_graphs_free(vc_ctx *ctx, vc_graphs *s)
 {
     if (!s) return VC_EINVAL;
-    if (ctx) hipSetDevice(ctx->device);
-    if (s->d_words) hipFree(s->d_words);
-    if (s->d_desc) hipFree(s->d_desc);
-    if (s->d_covers) hipFree(s->d_covers);
-    if (s->d_state) hipFree(s->d_state);
-    if (s->h_state) hipHostFree(s->h_state);
-    if (s->h_glob) hipHostFree(s->h_glob);
-    if (s->h_ws) hipHostFree(s->h_ws);
-    if (s->h_cov) hipHostFree(s->h_cov);
-    if (s->exec) hipGraphExecDestroy(s->exec);
-    if (s->graph) hipGraphDestroy(s->graph);
+    if (ctx) (void)hipSetDevice(ctx->device);
+    if (s->d_words) (void)hipFree(s->d_words);
+    if (s->d_desc) (void)hipFree(s->d_desc);
+    if (s->d_covers) (void)hipFree(s->d_covers);
+    if (s->d_state) (void)hipFree(s->d_state);
+    if (s->h_state) (void)hipHostFree(s->h_state);
+    if (s->h_glob) (void)hipHostFree(s->h_glob);
+    if (s->h_ws) (void)hipHostFree(s->h_ws);
+    if (s->h_cov) (void)hipHostFree(s->h_cov);
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
     delete s;
     return VC_OK;
 }
@@ -1605,7 +1696,7 @@ vc_status vc_occupancy(vc_ctx *ctx, const vc_graphs *set, const vc_params *param
                        int *use_global_memory)
 {
     if (!ctx || !set || !params) return VC_EINVAL;
-    hipSetDevice(ctx->device);
+    if (!hip_ok(hipSetDevice(ctx->device))) return VC_EDEVICE;
     std::vector<vc_job> jobs(set->count);
     for (int i = 0; i < set->count; ++i) {
         jobs[i] = vc_job{VC_MVC, 0u, set->h_desc[i].n, 0u, 0ull};
@@ -1615,7 +1706,7 @@ vc_status vc_occupancy(vc_ctx *ctx, const vc_graphs *set, const vc_params *param
     if (st != VC_OK) return st;
     if (blocks_per_cu) *blocks_per_cu = (int)pl.blocks_per_cu;
     if (block_dim) *block_dim = (int)pl.block_dim;
-    if (use_global_memory) *use_global_memory = pl.variant == kHbm32;
+    if (use_global_memory) *use_global_memory = pl.variant != kLds16;
     return VC_OK;
 }
 
@@ -1624,7 +1715,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
 {
     if (!ctx || !set_c || !jobs || !params || !results) return VC_EINVAL;
     vc_graphs *set = const_cast<vc_graphs *>(set_c);
-    hipSetDevice(ctx->device);
+    if (!hip_ok(hipSetDevice(ctx->device))) return VC_EDEVICE;
     for (int i = 0; i < set->count; ++i) {
         const vc_job &j = jobs[i];
         if (j.instance != VC_MVC && j.instance != VC_PVC) return VC_EINVAL;
@@ -1652,7 +1743,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         return VC_EDEVICE;
 
     if (ctx->scratch_bytes < pl.total) {
-        if (ctx->scratch) hipFree(ctx->scratch);
+        if (ctx->scratch) (void)hipFree(ctx->scratch);
         ctx->scratch = nullptr;
         ctx->scratch_bytes = 0;
         if (!hip_ok(hipMalloc(&ctx->scratch, pl.total))) return VC_ENOMEM;
@@ -1682,7 +1773,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
              hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_glob), sizeof(Global))) &&
              hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_cov), sizeof(uint32_t) * set->cover_words));
     if (ok && set->ws_cap < pl.workers) {
-        if (set->h_ws) hipHostFree(set->h_ws);
+        if (set->h_ws) (void)hipHostFree(set->h_ws);
         set->h_ws = nullptr;
         ok = hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_ws), sizeof(vcg::WorkerStats) * pl.workers));
         set->ws_cap = ok ? pl.workers : 0;
@@ -1726,8 +1817,8 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         key.scratch = ctx->scratch;
         key.h_ws = set->h_ws;
         if (!set->exec || std::memcmp(&key, &set->key, sizeof(key)) != 0) {
-            if (set->exec) hipGraphExecDestroy(set->exec);
-            if (set->graph) hipGraphDestroy(set->graph);
+            if (set->exec) (void)hipGraphExecDestroy(set->exec);
+            if (set->graph) (void)hipGraphDestroy(set->graph);
             set->exec = nullptr;
             set->graph = nullptr;
             if (!hip_ok(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal))) return VC_EDEVICE;
@@ -1735,11 +1826,11 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
             hipGraph_t g = nullptr;
             bool ended = hip_ok(hipStreamEndCapture(ctx->stream, &g));
             if (!cap || !ended || !g) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 return VC_EDEVICE;
             }
             if (!hip_ok(hipGraphInstantiate(&set->exec, g, nullptr, nullptr, 0))) {
-                hipGraphDestroy(g);
+                (void)hipGraphDestroy(g);
                 return VC_EDEVICE;
             }
             set->graph = g;
@@ -1753,7 +1844,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     }
     if (!hip_ok(hipStreamSynchronize(ctx->stream))) return VC_EDEVICE;
     float ms = 0.f;
-    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) ms = 0.f;
     const GState *hs = set->h_state;
     const Global &hg = *set->h_glob;
     const vcg::WorkerStats *wn = set->h_ws;
@@ -1796,7 +1887,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         info->block_dim = pl.block_dim;
         info->workers = pl.workers;
         info->blocks_per_cu = pl.blocks_per_cu;
-        info->use_global_memory = pl.variant == kHbm32;
+        info->use_global_memory = pl.variant != kLds16;
         info->list_size = pl.qsize;
         info->list_threshold = pl.qthreshold;
         info->stack_depth = pl.depth;
@@ -2116,7 +2207,7 @@ uint32_t server_collect(vc_server *s)
 
 void server_loop(vc_server *s)
 {
-    hipSetDevice(s->ctx->device);
+    (void)hipSetDevice(s->ctx->device);
     uint32_t idle_rounds = 0;
     for (;;) {
         uint32_t work = server_admit(s) + server_collect(s);
@@ -2133,7 +2224,7 @@ void server_loop(vc_server *s)
     }
     // let the kernel drain and exit
     __atomic_store_n(&s->host->shutdown, 1u, __ATOMIC_RELEASE);
-    hipStreamSynchronize(s->kstream);
+    (void)hipStreamSynchronize(s->kstream);
     server_collect(s);
     std::lock_guard<std::mutex> lk(s->mu);
     auto fail = [&](uint64_t ticket) {
@@ -2153,18 +2244,18 @@ void server_loop(vc_server *s)
 
 void server_free(vc_server *s)
 {
-    if (s->scratch) hipFree(s->scratch);
-    if (s->d_desc) hipFree(s->d_desc);
-    if (s->d_state) hipFree(s->d_state);
-    if (s->d_arena) hipFree(s->d_arena);
-    if (s->host) hipHostFree(s->host);
-    if (s->comp) hipHostFree(s->comp);
-    if (s->h_desc) hipHostFree(s->h_desc);
-    if (s->h_ring) hipHostFree(s->h_ring);
-    if (s->h_arena) hipHostFree(s->h_arena);
-    if (s->ev0) hipEventDestroy(s->ev0);
-    if (s->ev1) hipEventDestroy(s->ev1);
-    if (s->kstream) hipStreamDestroy(s->kstream);
+    if (s->scratch) (void)hipFree(s->scratch);
+    if (s->d_desc) (void)hipFree(s->d_desc);
+    if (s->d_state) (void)hipFree(s->d_state);
+    if (s->d_arena) (void)hipFree(s->d_arena);
+    if (s->host) (void)hipHostFree(s->host);
+    if (s->comp) (void)hipHostFree(s->comp);
+    if (s->h_desc) (void)hipHostFree(s->h_desc);
+    if (s->h_ring) (void)hipHostFree(s->h_ring);
+    if (s->h_arena) (void)hipHostFree(s->h_arena);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->kstream) (void)hipStreamDestroy(s->kstream);
     delete s;
 }
 
@@ -2184,7 +2275,7 @@ vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **
 {
     if (!ctx || !cfg || !out || cfg->max_vertices == 0) return VC_EINVAL;
     *out = nullptr;
-    hipSetDevice(ctx->device);
+    if (!hip_ok(hipSetDevice(ctx->device))) return VC_EDEVICE;
     vc_server *s = new vc_server();
     s->ctx = ctx;
     s->cfg = *cfg;
@@ -2246,7 +2337,7 @@ vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **
     ok = hip_ok(hipGetLastError()) && hip_ok(hipEventRecord(s->ev0, s->kstream)) &&
          launch_solve(s->pl, s->kstream, a) && hip_ok(hipEventRecord(s->ev1, s->kstream));
     if (!ok) {
-        hipStreamSynchronize(s->kstream);
+        (void)hipStreamSynchronize(s->kstream);
         server_free(s);
         return VC_EDEVICE;
     }
@@ -2349,7 +2440,7 @@ vc_status vc_server_stop(vc_server *s, vc_launch_info *info)
             info->steals += x.steals;
         }
         float ms = 0.f;
-        hipEventElapsedTime(&ms, s->ev0, s->ev1);
+        if (hipEventElapsedTime(&ms, s->ev0, s->ev1) != hipSuccess) ms = 0.f;
         info->kernel_ms = ms;
         double span = (double)ms * 1e-3 * (double)s->ctx->wall_khz * 1e3 * (double)s->pl.workers;
         info->idle_fraction = span > 0 ? (float)std::min(1.0, (double)idle / span) : 0.f;
@@ -2357,7 +2448,7 @@ vc_status vc_server_stop(vc_server *s, vc_launch_info *info)
         info->block_dim = s->pl.block_dim;
         info->workers = s->pl.workers;
         info->blocks_per_cu = s->pl.blocks_per_cu;
-        info->use_global_memory = s->pl.variant == kHbm32;
+        info->use_global_memory = s->pl.variant != kLds16;
         info->list_size = s->pl.qsize;
         info->list_threshold = s->pl.qthreshold;
         info->stack_depth = s->pl.depth;

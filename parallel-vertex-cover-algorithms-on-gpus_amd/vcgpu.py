@@ -274,6 +274,7 @@ class Params:
 
 
 VC_FLAG_GRAPH = 1
+VC_FLAG_WIDE = 2
 
 
 class GraphSet:

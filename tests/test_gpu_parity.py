@@ -161,6 +161,25 @@ def test_hbm_mode_matches_oracle(ctx):
         assert O.cover_is_valid(og, cover) and int(np.sum(cover)) == want
 
 
+@pytest.mark.parametrize("version", [1, 0])
+def test_wide_storage_matches_oracle(ctx, version):
+    """int32 degrees with 64-bit sketch sums (the storage of graphs with
+    n >= 32768), forced on small graphs; MVC and PVC at k = opt - 1, opt."""
+    for name, n, e in _medium_cases()[:8]:
+        og = O.from_edges(n, e)
+        want, _, _ = O.mvc(og)
+        g = vcgpu.csr_from_edges(n, e)
+        mvc, cover, r, info = vcgpu.solve_mvc(ctx, g, P(flags=vcgpu.VC_FLAG_WIDE, version=version))
+        assert info.use_global_memory == 1
+        assert r.solved and mvc == want, (name, mvc, want)
+        assert O.cover_is_valid(og, cover) and int(np.sum(cover)) == want
+        for k in (k for k in (want - 1, want) if k >= 0):
+            found, pcover, pr, _ = vcgpu.solve_pvc(ctx, g, k, P(flags=vcgpu.VC_FLAG_WIDE, version=version))
+            assert pr.solved and found == (k >= want), (name, k)
+            if found:
+                assert O.cover_is_valid(og, pcover) and int(np.sum(pcover)) <= k
+
+
 def _known():
     import json
     import os
