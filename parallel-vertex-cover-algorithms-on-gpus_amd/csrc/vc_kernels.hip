@@ -190,9 +190,11 @@ __device__ __forceinline__ bool past_deadline(const SolveArgs &a)
 // sums suffice; wider graphs keep a 64-bit square sum.
 template <typename DegT> struct SketchOf {
     using s2_t = uint32_t;
+    using cand_t = uint16_t;  // vertex ids below 2^15
 };
 template <> struct SketchOf<int32_t> {
     using s2_t = unsigned long long;
+    using cand_t = uint32_t;
 };
 // bytes of one node (degrees + sketch) per vertex
 template <typename DegT> constexpr uint32_t node_bytes_per_vertex()
@@ -204,9 +206,12 @@ template <typename DegT, bool HBM>
 struct Node {
     using deg_t = DegT;
     using s2_t = typename SketchOf<DegT>::s2_t;
+    using cand_t = typename SketchOf<DegT>::cand_t;
     DegT *deg;           // [npad]: degree, -1 = in cover
     uint32_t *s1;        // [npad]: sum of live neighbour ids
     s2_t *s2;            // [npad]: sum of squared live neighbour ids
+    cand_t *cand;        // [ncap]: vertices whose degree fell to 1 since the last leaf pass
+    uint32_t ncap;
     uint32_t *mark;      // [mwords] bitset, all zero between passes
     uint32_t *list;      // [lcap]: vertices marked in the current pass
     int *xs;             // [3 * 64] scratch of apply_list: edge offsets, list bases, list vertices
@@ -248,27 +253,46 @@ __device__ __forceinline__ uint32_t append(const N &w, uint32_t L, uint32_t cand
 // degrees the half-word is addressed by adding -1 or -(1 << 16).  A decremented
 // vertex is live with degree >= 1, so no borrow crosses into the other half.
 template <class N>
-__device__ __forceinline__ void dec_degree(const N &w, uint32_t t)
+__device__ __forceinline__ int dec_degree(const N &w, uint32_t t)
 {
     using D = typename N::deg_t;
     if constexpr (sizeof(D) == 2) {
         uint32_t *word = reinterpret_cast<uint32_t *>(w.deg) + (t >> 1);
         uint32_t delta = (t & 1u) ? 0xFFFF0000u : 0xFFFFFFFFu;
-        __hip_atomic_fetch_add(word, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        const uint32_t old = __hip_atomic_fetch_add(word, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        return (int)(int16_t)(uint16_t)((t & 1u) ? (old >> 16) : old);
     } else {
-        __hip_atomic_fetch_add(reinterpret_cast<int32_t *>(w.deg) + t, -1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+        return __hip_atomic_fetch_add(reinterpret_cast<int32_t *>(w.deg) + t, -1, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
 }
 
-// Live vertex t loses its live neighbour u (several lanes may hit the same t).
+// Leaf candidates: every vertex whose degree falls from 2 to 1 is appended
+// (lanes with t == kNone add nothing).  Degrees never grow along a path, so
+// a vertex enters at most once between two leaf passes; a count beyond ncap
+// (never expected) makes the next leaf pass scan every vertex.  Returns the
+// new count (wave-uniform; every lane must call).
 template <class N>
-__device__ __forceinline__ void drop_neighbour(const N &w, uint32_t t, uint32_t u)
+__device__ __forceinline__ uint32_t add_leaf_cands(const N &w, uint32_t nc, uint32_t t)
+{
+    const uint64_t m = __ballot(t != kNone);
+    if (t != kNone) {
+        const uint32_t pos = nc + lanes_below(m);
+        if (pos < w.ncap) w.cand[pos] = (typename N::cand_t)t;
+    }
+    return nc + (uint32_t)__popcll(m);
+}
+
+// Live vertex t loses its live neighbour u (several lanes may hit the same t).
+// Returns t's degree before the update.
+template <class N>
+__device__ __forceinline__ int drop_neighbour(const N &w, uint32_t t, uint32_t u)
 {
     using S2 = typename N::s2_t;
-    dec_degree(w, t);
+    const int old = dec_degree(w, t);
     __hip_atomic_fetch_add(&w.s1[t], 0u - u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     __hip_atomic_fetch_add(&w.s2[t], (S2)0 - (S2)u * (S2)u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    return old;
 }
 
 // Exact integer square root of a perfect square below 2^30 (32-bit sums) or
@@ -279,15 +303,22 @@ __device__ __forceinline__ unsigned long long isqrt_round(unsigned long long x)
     return (unsigned long long)(__builtin_sqrt((double)x) + 0.5);
 }
 
-// The two live neighbours n1 < n2 of a degree-2 vertex from its sketch.
-// False when the sketch does not decode to two distinct vertices below n
-// (impossible for a consistent node).
+// Top bit of s2 on a degree-2 vertex: its two neighbours were found not to
+// be adjacent.  While deg(v) == 2 its neighbour pair cannot change, and
+// degrees never grow along a path of the tree, so the result holds in every
+// descendant node; once deg(v) < 2 the bit is never read.  Below the bit,
+// n1^2 + n2^2 < 2^31 (int16 degrees) or 2^63, so it never carries.
+template <typename S2> constexpr S2 kNotTriangle = (S2)1 << (8 * sizeof(S2) - 1);
+
+// The two live neighbours n1 < n2 of a degree-2 vertex from its sketch
+// (s2 without the kNotTriangle bit).  False when the sketch does not decode
+// to two distinct vertices below n (impossible for a consistent node).
 template <class N>
-__device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, uint32_t &n1, uint32_t &n2)
+__device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, typename N::s2_t q, uint32_t &n1,
+                                               uint32_t &n2)
 {
     using S2 = typename N::s2_t;
     const uint32_t s = w.s1[v];
-    const S2 q = w.s2[v];
     const S2 x = (S2)2 * q - (S2)s * (S2)s;
     const S2 r = isqrt_round(x);
     if (r * r != x || r == 0 || r > (S2)s) return false;
@@ -304,7 +335,7 @@ __device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, uint32_t 
 // entry that owns its edge with a 6-step shuffle search), so one round of
 // global loads serves 64 edges whatever the degrees.
 template <class N>
-__device__ void apply_list(const N &w, uint32_t L)
+__device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
 {
     w.sync();
     for (uint32_t i = lane_id(); i < L; i += kWave) w.deg[w.list[i]] = -1;
@@ -328,16 +359,21 @@ __device__ void apply_list(const N &w, uint32_t L)
         const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
         w.sync();
 #pragma unroll 1
-        for (int x = lane; x < total; x += kWave) {
-            // owner: last entry j < cnt with xoff[j] <= x
-            int pos = 0;
+        for (int x0 = 0; x0 < total; x0 += kWave) {
+            const int x = x0 + lane;
+            uint32_t leaf = kNone;
+            if (x < total) {
+                // owner: last entry j < cnt with xoff[j] <= x
+                int pos = 0;
 #pragma unroll 1
-            for (int step = 32; step >= 1; step >>= 1) {
-                const int cand = pos + step;
-                if (cand < (int)cnt && xoff[cand] <= x) pos = cand;
+                for (int step = 32; step >= 1; step >>= 1) {
+                    const int cand = pos + step;
+                    if (cand < (int)cnt && xoff[cand] <= x) pos = cand;
+                }
+                const uint32_t t = w.dst[xbase[pos] + (x - xoff[pos])];
+                if (w.deg[t] != -1 && drop_neighbour(w, t, (uint32_t)xvert[pos]) == 2) leaf = t;
             }
-            const uint32_t t = w.dst[xbase[pos] + (x - xoff[pos])];
-            if (w.deg[t] != -1) drop_neighbour(w, t, (uint32_t)xvert[pos]);
+            nc = add_leaf_cands(w, nc, leaf);
         }
         w.sync();
     }
@@ -354,7 +390,6 @@ struct Lanes {
     int s;        // sum of live degrees
     int md;       // max degree of the lane's vertices
     uint32_t mv;  // lowest index with degree md in the lane
-    uint32_t two; // some vertex of the lane has degree 2
 };
 
 __device__ __forceinline__ void lanes_reset(Lanes &st)
@@ -362,7 +397,6 @@ __device__ __forceinline__ void lanes_reset(Lanes &st)
     st.s = 0;
     st.md = 0;
     st.mv = kNone;
-    st.two = 0u;
 }
 
 // Vertices reach a lane in increasing order, so a strict > keeps the lowest
@@ -374,36 +408,59 @@ __device__ __forceinline__ void lanes_add(Lanes &st, int dv, uint32_t v)
         st.md = dv;
         st.mv = v;
     }
-    st.two |= (uint32_t)(dv == 2);
 }
 
 // Leaf rule (src/helperFunctions.cuh:229-296): for every degree-1 vertex v
 // mark its live neighbour u when deg(u) > 1, or deg(u) == 1 and u < v (so an
 // isolated edge loses exactly one end).  Repeat until nothing is marked.
+// The reference scans every vertex in every pass.  Here a pass looks at the
+// candidate list only: a node enters the rules with no degree-1 vertex
+// (its parent ended at the rules' fixpoint) except at a root, and every later
+// degree-1 vertex got there by a decrement that listed it.  So a pass over
+// the candidates marks exactly what a full scan would.  Full scans are kept
+// for roots (full = true) and after a pass whose marks hit the list capacity
+// (the unmarked rest is found by the next full pass, as in the reference).
 template <class N>
-__device__ uint32_t rule_leaf(const N &w, Lanes &st)
+__device__ __forceinline__ uint32_t leaf_mark(const N &w, uint32_t v, int dv)
+{
+    if (dv == 1) {
+        const uint32_t u = w.s1[v];
+        if (u < w.n) {
+            const int du = w.deg[u];
+            if (du > 1 || (du == 1 && u < v)) return u;
+        }
+    }
+    return kNone;
+}
+
+template <class N>
+__device__ uint32_t rule_leaf(const N &w, uint32_t &nc, bool &full)
 {
     uint32_t total = 0;
     for (;;) {
         uint32_t L = 0;
-        lanes_reset(st);
-        for (uint32_t base = 0; base < w.n; base += kWave) {
-            uint32_t v = base + lane_id();
-            uint32_t cand = kNone;
-            int dv = v < w.n ? (int)w.deg[v] : -1;
-            lanes_add(st, dv, v);
-            if (dv == 1) {
-                const uint32_t u = w.s1[v];
-                if (u < w.n) {
-                    const int du = w.deg[u];
-                    if (du > 1 || (du == 1 && u < v)) cand = u;
-                }
+        if (full || nc > w.ncap) {
+            for (uint32_t base = 0; base < w.n; base += kWave) {
+                const uint32_t v = base + lane_id();
+                const int dv = v < w.n ? (int)w.deg[v] : -1;
+                L = append(w, L, leaf_mark(w, v, dv), true);
             }
-            L = append(w, L, cand, true);
+        } else {
+            for (uint32_t i0 = 0; i0 < nc; i0 += kWave) {
+                const uint32_t i = i0 + lane_id();
+                const uint32_t v = i < nc ? (uint32_t)w.cand[i] : kNone;
+                const int dv = v != kNone ? (int)w.deg[v] : -1;
+                L = append(w, L, leaf_mark(w, v, dv), true);
+            }
         }
-        if (L == 0) break;
-        apply_list(w, L);
+        nc = 0;  // consumed; the apply below lists the next candidates
+        if (L == 0) {
+            full = false;
+            break;
+        }
+        apply_list(w, L, nc);
         total += L;
+        full = L >= w.lcap;
     }
     return total;
 }
@@ -429,7 +486,7 @@ __device__ __forceinline__ bool adj_has(const uint32_t *dst, uint32_t lo, uint32
 // cover.  Skipped when a neighbour has degree 1, or degree 2 with a lower
 // index, as in the reference.  All triangles of a pass are applied together.
 template <class N>
-__device__ uint32_t rule_triangle(const N &w, Lanes &st)
+__device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
 {
     uint32_t total = 0;
     for (;;) {
@@ -440,22 +497,29 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st)
             uint32_t c1 = kNone, c2 = kNone;
             int dv = v < w.n ? (int)w.deg[v] : -1;
             lanes_add(st, dv, v);
+            using S2 = typename N::s2_t;
             uint32_t n1, n2;
-            if (dv == 2 && two_neighbours(w, v, n1, n2)) {
+            S2 q = 0;
+            if (dv == 2) q = w.s2[v];
+            if (dv == 2 && (q & kNotTriangle<S2>) == 0 && two_neighbours(w, v, q, n1, n2)) {
                 // skipped when a neighbour has degree 1, or degree 2 and a
                 // lower index (src/helperFunctions.cuh:379-385)
                 const int d1 = w.deg[n1], d2 = w.deg[n2];
                 const bool skip = d1 == 1 || (d1 == 2 && n1 < v) || d2 == 1 || (d2 == 2 && n2 < v);
-                if (!skip && adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
-                    c1 = n1;
-                    c2 = n2;
+                if (!skip) {
+                    if (adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
+                        c1 = n1;
+                        c2 = n2;
+                    } else {
+                        w.s2[v] = q | kNotTriangle<S2>;
+                    }
                 }
             }
             L = append(w, L, c1, true);
             L = append(w, L, c2, true);
         }
         if (L == 0) break;
-        apply_list(w, L);
+        apply_list(w, L, nc);
         total += L;
     }
     return total;
@@ -464,7 +528,7 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st)
 // High-degree rule (src/helperFunctions.cuh:299-358): v goes into the cover
 // when deg(v) + d + (deleted so far in this call) >= T.
 template <class N>
-__device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st)
+__device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st, uint32_t &nc)
 {
     uint32_t total = 0;
     for (;;) {
@@ -481,7 +545,7 @@ __device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st)
             L = append(w, L, cand, false);
         }
         if (L == 0) break;
-        apply_list(w, L);
+        apply_list(w, L, nc);
         total += L;
     }
     return total;
@@ -495,32 +559,32 @@ struct Summary {
 
 // Rule fixpoint in the order of src/GlobalWorkList.cuh:107-130.  Returns the
 // new number of vertices in the cover and, in *sum, the node's edge count and
-// max-degree vertex.  Every rule call ends with a scan that marked nothing,
-// so the statistics of the latest scan are exact for the current degrees:
-//  * the triangle rule is not scanned when no vertex has degree 2 (it could
-//    not mark anything);
-//  * the high-degree rule is not scanned when max degree + d < T (its first
-//    scan could not mark anything);
-//  * the summary needs no scan of its own.
-// The rules that run, their order and their results are unchanged.
+// max-degree vertex.  nc / full carry the leaf candidates (see rule_leaf).
+// The triangle rule scans every vertex whenever it runs (always in the
+// first round), and its last scan marks nothing, so the statistics gathered
+// by the latest triangle or high-degree scan are exact for the current
+// degrees: the high-degree rule is not scanned when max degree + d < T (its
+// first scan could not mark anything), and the summary needs no scan of its
+// own.  The rules that run, their order and their results are unchanged.
 template <class N>
-__device__ uint32_t reduce_node(const N &w, uint32_t d, uint32_t T, Summary *sum)
+__device__ uint32_t reduce_node(const N &w, uint32_t d, uint32_t T, Summary *sum, uint32_t &nc, bool &full)
 {
     uint32_t it = 0, nl, nt = 0, nh = 0;
     Lanes st;
     do {
-        nl = rule_leaf(w, st);
+        nl = rule_leaf(w, nc, full);
         d += nl;
         nt = 0;
-        if ((it == 0 || nl > 0 || nh > 0) && __ballot(st.two != 0u) != 0ull) {
-            nt = rule_triangle(w, st);
+        if (it == 0 || nl > 0 || nh > 0) {
+            nt = rule_triangle(w, st, nc);
             d += nt;
         }
         nh = 0;
         if (it == 0 || nl > 0 || nt > 0) {
+            // the triangle rule ran in this round: st is exact
             const int md = wave_max(st.md);
             if ((uint32_t)md + d >= T) {
-                nh = rule_high(w, d, T, st);
+                nh = rule_high(w, d, T, st, nc);
                 d += nh;
             }
         }
@@ -536,22 +600,28 @@ __device__ uint32_t reduce_node(const N &w, uint32_t d, uint32_t T, Summary *sum
 // Left child in place: v into the cover (deleteMaxDegreeVertex,
 // src/helperFunctions.cuh:200-215).
 template <class N>
-__device__ void take_vertex(const N &w, uint32_t v)
+__device__ void take_vertex(const N &w, uint32_t v, uint32_t &nc)
 {
     w.sync();
     if (lane_id() == 0) w.deg[v] = -1;
     w.sync();
     using S2 = typename N::s2_t;
-    uint32_t e = w.src[v + 1];
-    for (uint32_t x = w.src[v] + lane_id(); x < e; x += kWave) {
-        uint32_t t = w.dst[x];
-        int dt = w.deg[t];
-        if (dt != -1) {
-            // the neighbours of one vertex are distinct: plain updates
-            w.deg[t] = (typename N::deg_t)(dt - 1);
-            w.s1[t] -= v;
-            w.s2[t] -= (S2)v * (S2)v;
+    const uint32_t b = w.src[v], e = w.src[v + 1];
+    for (uint32_t x0 = b; x0 < e; x0 += kWave) {
+        const uint32_t x = x0 + lane_id();
+        uint32_t leaf = kNone;
+        if (x < e) {
+            const uint32_t t = w.dst[x];
+            const int dt = w.deg[t];
+            if (dt != -1) {
+                // the neighbours of one vertex are distinct: plain updates
+                w.deg[t] = (typename N::deg_t)(dt - 1);
+                w.s1[t] -= v;
+                w.s2[t] -= (S2)v * (S2)v;
+                if (dt == 2) leaf = t;
+            }
         }
+        nc = add_leaf_cands(w, nc, leaf);
     }
     w.sync();
 }
@@ -561,7 +631,7 @@ __device__ void take_vertex(const N &w, uint32_t v)
 // rounds of at most lcap vertices.  Returns how many vertices were added
 // (deg(v) before the call).
 template <class N>
-__device__ uint32_t take_neighbours(const N &w, uint32_t v)
+__device__ uint32_t take_neighbours(const N &w, uint32_t v, uint32_t &nc)
 {
     uint32_t total = 0;
     const uint32_t b = w.src[v], e = w.src[v + 1];
@@ -577,7 +647,7 @@ __device__ uint32_t take_neighbours(const N &w, uint32_t v)
             L = append(w, L, cand, false);
         }
         if (L == 0) break;
-        apply_list(w, L);
+        apply_list(w, L, nc);
         total += L;
     }
     return total;
@@ -1001,7 +1071,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
     if (wid >= a.workers) return;
 
     // node storage: deg [npad DegT] | s1 [npad u32] | s2 [npad] | list [lcap u32] | mark [mwords u32] |
-    // xs [192 i32]; a stack or worklist slot holds deg | s1 | s2
+    // xs [192 i32] | cand [npad]; a stack or worklist slot holds deg | s1 | s2
     char *store = HBM ? a.hbm_nodes + (size_t)wid * a.worker_bytes
                       : reinterpret_cast<char *>(smem4) + (size_t)wib * a.worker_bytes;
     const uint32_t slot_bytes = a.npad * node_bytes_per_vertex<DegT>();
@@ -1012,6 +1082,8 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
     w.list = reinterpret_cast<uint32_t *>(store + slot_bytes);
     w.mark = w.list + a.lcap;
     w.xs = reinterpret_cast<int *>(w.mark + a.mwords);
+    w.cand = reinterpret_cast<typename Node<DegT, HBM>::cand_t *>(w.xs + 3 * kWave);
+    w.ncap = a.npad;
     w.src = nullptr;
     w.dst = nullptr;
     w.n = 0;
@@ -1030,6 +1102,8 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
     bool have = false;
     uint32_t d = 0;
     uint32_t path = 0, descend = 0;
+    uint32_t nc = 0;    // leaf candidates listed since the last leaf pass
+    bool full = true;   // the next leaf pass must scan every vertex
     uint64_t pending = 0, total_nodes = 0, idle_ticks = 0, donations = 0, steals = 0;
 
     for (;;) {
@@ -1047,7 +1121,9 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                 d = uni(m.x);
                 uint32_t v = uni(m.y);
                 --top;
-                d += take_neighbours(w, v);
+                nc = 0;
+                full = false;
+                d += take_neighbours(w, v, nc);
                 have = true;
             } else {
                 if (g != kNone) {
@@ -1071,12 +1147,15 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                 w.n = gd.n;
                 w.src = a.gbuf + gd.off_src;
                 w.dst = a.gbuf + gd.off_dst;
+                nc = 0;
                 if (q.kind == 1) {
                     d = q.d;
-                    if (q.v != kNone) d += take_neighbours(w, q.v);
+                    full = q.v == kNone;
+                    if (q.v != kNone) d += take_neighbours(w, q.v, nc);
                     descend = 0;
                 } else {
                     init_root(w, reinterpret_cast<const int32_t *>(a.gbuf + gd.off_deg), a.npad);
+                    full = true;
                     d = 0;
                     if (a.version == 1) {
                         descend = 0;
@@ -1113,7 +1192,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
         const bool mvc = gd.instance == 0;
         uint32_t T = mvc ? (uint32_t)uni((uint32_t)ld_rlx(&gs->best)) : gd.k + 1u;
         Summary s;
-        d = reduce_node(w, d, T, &s);
+        d = reduce_node(w, d, T, &s, nc, full);
 
         bool prune;
         if (mvc) {
@@ -1139,10 +1218,10 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
             uint32_t bit = (path >> (a.start_depth - descend)) & 1u;
             --descend;
             if (bit == 0u) {
-                take_vertex(w, v);
+                take_vertex(w, v, nc);
                 d += 1;
             } else {
-                d += take_neighbours(w, v);
+                d += take_neighbours(w, v, nc);
             }
             continue;
         }
@@ -1201,7 +1280,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                 }
             }
         }
-        take_vertex(w, v);
+        take_vertex(w, v, nc);
         d += 1;
     }
     if (lane_id() == 0) {
@@ -1380,7 +1459,7 @@ struct Plan {
 };
 
 // Per-worker node storage for a degree type: deg [npad] | s1 [npad] | s2 [npad] | list [lcap] |
-// mark [mwords] | xs [192].  A stack / worklist slot is deg | s1 | s2.
+// mark [mwords] | xs [192] | cand [npad].  A stack / worklist slot is deg | s1 | s2.
 void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes)
 {
     pl.deg_bytes = deg_bytes;
@@ -1389,7 +1468,9 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes)
                                                : vcg::node_bytes_per_vertex<int32_t>());
     pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), 1024u);
     pl.mwords = round_up((nmax + 31u) / 32u, 4u);
-    pl.worker_bytes = round_up(pl.slot_bytes + 4u * pl.lcap + 4u * pl.mwords + 4u * 3u * vcg::kWave, 64u);
+    const uint32_t cand_bytes = pl.npad * (deg_bytes == 2u ? 2u : 4u);
+    pl.worker_bytes =
+        round_up(pl.slot_bytes + 4u * pl.lcap + 4u * pl.mwords + 4u * 3u * vcg::kWave + cand_bytes, 64u);
 }
 
 // Launch shape shared by the batch solve and the server: node storage,
