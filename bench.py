@@ -346,7 +346,9 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=1000.0, help="served: Poisson arrival rate (requests/s)")
     ap.add_argument("--small-blocks", type=int, default=256 * 5, help="served: workgroups of the small-graph server")
-    ap.add_argument("--large-blocks", type=int, default=64, help="served: one-wave workgroups of the large-graph server")
+    ap.add_argument("--large-blocks", type=int, default=16,
+                    help="served: one-wave workgroups of the large-graph server (each holds an exact-depth stack, "
+                         "about 2 GB for BA n=20000)")
     ap.add_argument("--large-budget", type=int, default=1 << 16, help="served: node budget of a large request")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--base-seed", type=int, default=1000)

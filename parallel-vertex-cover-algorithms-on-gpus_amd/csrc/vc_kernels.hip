@@ -179,15 +179,22 @@ __device__ __forceinline__ bool past_deadline(const SolveArgs &a)
 
 // Neighbour sketch of a tree node.  Beside its degree (number of live
 // neighbours; -1 = in the cover) every vertex v keeps
-//   s1[v] = sum of the ids of its live neighbours      (mod 2^32)
-//   s2[v] = sum of the squared ids of its live neighbours (mod 2^32 or 2^64)
-// so a degree-1 vertex's neighbour is s1[v], and a degree-2 vertex's two
+//   s1(v) = sum of the ids of its live neighbours
+//   s2(v) = sum of the squared ids of its live neighbours
+// so a degree-1 vertex's neighbour is s1(v), and a degree-2 vertex's two
 // neighbours n1 < n2 follow from n1 + n2 = s1 and n1^2 + n2^2 = s2:
 // (n2 - n1)^2 = 2 s2 - s1^2.  That replaces the reference's walk over the
 // whole original adjacency list to find the live neighbours
-// (src/helperFunctions.cuh:240-252, 372-392) with two local reads.  With
-// int16 degrees (n < 2^15) the decoded values are below 2^31, so 32-bit
-// sums suffice; wider graphs keep a 64-bit square sum.
+// (src/helperFunctions.cuh:240-252, 372-392) with local reads.
+//
+// Layouts (one stack / worklist slot = the arrays up to s2):
+//  * int16 graphs (n < 2^15), 8 bytes per vertex: one 32-bit word
+//    [s1 mod 2^16 : 16 | degree : 16] and a 32-bit s2.  The values decoded
+//    (one id, the sum of two ids, n1^2 + n2^2 < 2^31) fit, and one 32-bit
+//    atomic updates degree and s1 together: a degree being decremented is
+//    >= 1, so no borrow crosses into s1, whose own wrap-around leaves the
+//    word.
+//  * int32 graphs, 16 bytes per vertex: int32 degrees, 32-bit s1, 64-bit s2.
 template <typename DegT> struct SketchOf {
     using s2_t = uint32_t;
     using cand_t = uint16_t;  // vertex ids below 2^15
@@ -199,17 +206,19 @@ template <> struct SketchOf<int32_t> {
 // bytes of one node (degrees + sketch) per vertex
 template <typename DegT> constexpr uint32_t node_bytes_per_vertex()
 {
-    return (uint32_t)(sizeof(DegT) + sizeof(uint32_t) + sizeof(typename SketchOf<DegT>::s2_t));
+    return sizeof(DegT) == 2 ? 8u : 16u;
 }
 
 template <typename DegT, bool HBM>
 struct Node {
-    using deg_t = DegT;
+    static constexpr bool kPacked = sizeof(DegT) == 2;
     using s2_t = typename SketchOf<DegT>::s2_t;
     using cand_t = typename SketchOf<DegT>::cand_t;
-    DegT *deg;           // [npad]: degree, -1 = in cover
-    uint32_t *s1;        // [npad]: sum of live neighbour ids
-    s2_t *s2;            // [npad]: sum of squared live neighbour ids
+    char *base;          // start of the node state (what a slot copy moves)
+    uint32_t *dw;        // [npad] packed [s1:16 | degree:16] (int16 layout)
+    int32_t *deg32;      // [npad] degrees (int32 layout)
+    uint32_t *s1w;       // [npad] s1 (int32 layout)
+    s2_t *s2;            // [npad] sum of squared live neighbour ids
     cand_t *cand;        // [ncap]: vertices whose degree fell to 1 since the last leaf pass
     uint32_t ncap;
     uint32_t *mark;      // [mwords] bitset, all zero between passes
@@ -221,6 +230,67 @@ struct Node {
     uint32_t lcap;
 
     __device__ __forceinline__ void sync() const { nsync<HBM>(); }
+
+    __device__ __forceinline__ int deg(uint32_t v) const
+    {
+        if constexpr (kPacked)
+            return (int)(int16_t)(uint16_t)dw[v];
+        else
+            return deg32[v];
+    }
+    __device__ __forceinline__ uint32_t s1(uint32_t v) const
+    {
+        if constexpr (kPacked)
+            return dw[v] >> 16;
+        else
+            return s1w[v];
+    }
+    // degree := -1 (v joins the cover); its sketch is not read again
+    __device__ __forceinline__ void remove(uint32_t v) const
+    {
+        if constexpr (kPacked)
+            reinterpret_cast<uint16_t *>(dw)[2u * v] = 0xFFFFu;
+        else
+            deg32[v] = -1;
+    }
+    // Live vertex t loses its live neighbour u; several lanes may hit the
+    // same t.  Returns t's degree before the update.
+    __device__ __forceinline__ int drop(uint32_t t, uint32_t u) const
+    {
+        int old;
+        if constexpr (kPacked) {
+            const uint32_t o = __hip_atomic_fetch_add(&dw[t], 0u - ((u << 16) + 1u), __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WAVEFRONT);
+            old = (int)(int16_t)(uint16_t)o;
+        } else {
+            old = __hip_atomic_fetch_add(&deg32[t], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            __hip_atomic_fetch_add(&s1w[t], 0u - u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+        __hip_atomic_fetch_add(&s2[t], (s2_t)0 - (s2_t)u * (s2_t)u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+        return old;
+    }
+    // The same with t updated by this lane only (dt = its current degree).
+    __device__ __forceinline__ void drop_plain(uint32_t t, uint32_t u, int dt) const
+    {
+        if constexpr (kPacked) {
+            dw[t] -= (u << 16) + 1u;
+        } else {
+            deg32[t] = dt - 1;
+            s1w[t] -= u;
+        }
+        s2[t] -= (s2_t)u * (s2_t)u;
+    }
+    __device__ __forceinline__ void init_vertex(uint32_t v, int d, uint32_t sum, s2_t sq) const
+    {
+        if constexpr (kPacked) {
+            dw[v] = (sum << 16) | (uint32_t)(uint16_t)(int16_t)d;
+        } else {
+            deg32[v] = d;
+            s1w[v] = sum;
+        }
+        s2[v] = sq;
+    }
 };
 
 // Appends cand (kNone = nothing) of every lane to the mark list; with dedupe
@@ -248,25 +318,6 @@ __device__ __forceinline__ uint32_t append(const N &w, uint32_t L, uint32_t cand
     return min(L + (uint32_t)__popcll(m), w.lcap);
 }
 
-// Decrements a live vertex's degree.  Several lanes may hit the same vertex,
-// so the update is an atomic on the containing 32-bit word; for int16
-// degrees the half-word is addressed by adding -1 or -(1 << 16).  A decremented
-// vertex is live with degree >= 1, so no borrow crosses into the other half.
-template <class N>
-__device__ __forceinline__ int dec_degree(const N &w, uint32_t t)
-{
-    using D = typename N::deg_t;
-    if constexpr (sizeof(D) == 2) {
-        uint32_t *word = reinterpret_cast<uint32_t *>(w.deg) + (t >> 1);
-        uint32_t delta = (t & 1u) ? 0xFFFF0000u : 0xFFFFFFFFu;
-        const uint32_t old = __hip_atomic_fetch_add(word, delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-        return (int)(int16_t)(uint16_t)((t & 1u) ? (old >> 16) : old);
-    } else {
-        return __hip_atomic_fetch_add(reinterpret_cast<int32_t *>(w.deg) + t, -1, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WAVEFRONT);
-    }
-}
-
 // Leaf candidates: every vertex whose degree falls from 2 to 1 is appended
 // (lanes with t == kNone add nothing).  Degrees never grow along a path, so
 // a vertex enters at most once between two leaf passes; a count beyond ncap
@@ -284,20 +335,14 @@ __device__ __forceinline__ uint32_t add_leaf_cands(const N &w, uint32_t nc, uint
 }
 
 // Live vertex t loses its live neighbour u (several lanes may hit the same t).
-// Returns t's degree before the update.
-template <class N>
-__device__ __forceinline__ int drop_neighbour(const N &w, uint32_t t, uint32_t u)
-{
-    using S2 = typename N::s2_t;
-    const int old = dec_degree(w, t);
-    __hip_atomic_fetch_add(&w.s1[t], 0u - u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    __hip_atomic_fetch_add(&w.s2[t], (S2)0 - (S2)u * (S2)u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    return old;
-}
-
 // Exact integer square root of a perfect square below 2^30 (32-bit sums) or
 // 2^62 (64-bit sums); the caller checks r * r == x.
-__device__ __forceinline__ uint32_t isqrt_round(uint32_t x) { return (uint32_t)(__builtin_sqrtf((float)x) + 0.5f); }
+// (the hardware square root is within 1 ulp: rounding gives the exact root
+// of a perfect square below 2^30)
+__device__ __forceinline__ uint32_t isqrt_round(uint32_t x)
+{
+    return (uint32_t)(__builtin_amdgcn_sqrtf((float)x) + 0.5f);
+}
 __device__ __forceinline__ unsigned long long isqrt_round(unsigned long long x)
 {
     return (unsigned long long)(__builtin_sqrt((double)x) + 0.5);
@@ -318,7 +363,7 @@ __device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, typename 
                                                uint32_t &n2)
 {
     using S2 = typename N::s2_t;
-    const uint32_t s = w.s1[v];
+    const uint32_t s = w.s1(v);
     const S2 x = (S2)2 * q - (S2)s * (S2)s;
     const S2 r = isqrt_round(x);
     if (r * r != x || r == 0 || r > (S2)s) return false;
@@ -338,7 +383,7 @@ template <class N>
 __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
 {
     w.sync();
-    for (uint32_t i = lane_id(); i < L; i += kWave) w.deg[w.list[i]] = -1;
+    for (uint32_t i = lane_id(); i < L; i += kWave) w.remove(w.list[i]);
     w.sync();
     const int lane = (int)lane_id();
     int *const xoff = w.xs;              // exclusive edge offset of list entry j
@@ -371,7 +416,7 @@ __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
                     if (cand < (int)cnt && xoff[cand] <= x) pos = cand;
                 }
                 const uint32_t t = w.dst[xbase[pos] + (x - xoff[pos])];
-                if (w.deg[t] != -1 && drop_neighbour(w, t, (uint32_t)xvert[pos]) == 2) leaf = t;
+                if (w.deg(t) != -1 && w.drop(t, (uint32_t)xvert[pos]) == 2) leaf = t;
             }
             nc = add_leaf_cands(w, nc, leaf);
         }
@@ -424,9 +469,9 @@ template <class N>
 __device__ __forceinline__ uint32_t leaf_mark(const N &w, uint32_t v, int dv)
 {
     if (dv == 1) {
-        const uint32_t u = w.s1[v];
+        const uint32_t u = w.s1(v);
         if (u < w.n) {
-            const int du = w.deg[u];
+            const int du = w.deg(u);
             if (du > 1 || (du == 1 && u < v)) return u;
         }
     }
@@ -442,14 +487,14 @@ __device__ uint32_t rule_leaf(const N &w, uint32_t &nc, bool &full)
         if (full || nc > w.ncap) {
             for (uint32_t base = 0; base < w.n; base += kWave) {
                 const uint32_t v = base + lane_id();
-                const int dv = v < w.n ? (int)w.deg[v] : -1;
+                const int dv = v < w.n ? w.deg(v) : -1;
                 L = append(w, L, leaf_mark(w, v, dv), true);
             }
         } else {
             for (uint32_t i0 = 0; i0 < nc; i0 += kWave) {
                 const uint32_t i = i0 + lane_id();
                 const uint32_t v = i < nc ? (uint32_t)w.cand[i] : kNone;
-                const int dv = v != kNone ? (int)w.deg[v] : -1;
+                const int dv = v != kNone ? w.deg(v) : -1;
                 L = append(w, L, leaf_mark(w, v, dv), true);
             }
         }
@@ -495,7 +540,7 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
         for (uint32_t base = 0; base < w.n; base += kWave) {
             uint32_t v = base + lane_id();
             uint32_t c1 = kNone, c2 = kNone;
-            int dv = v < w.n ? (int)w.deg[v] : -1;
+            int dv = v < w.n ? w.deg(v) : -1;
             lanes_add(st, dv, v);
             using S2 = typename N::s2_t;
             uint32_t n1, n2;
@@ -504,7 +549,7 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
             if (dv == 2 && (q & kNotTriangle<S2>) == 0 && two_neighbours(w, v, q, n1, n2)) {
                 // skipped when a neighbour has degree 1, or degree 2 and a
                 // lower index (src/helperFunctions.cuh:379-385)
-                const int d1 = w.deg[n1], d2 = w.deg[n2];
+                const int d1 = w.deg(n1), d2 = w.deg(n2);
                 const bool skip = d1 == 1 || (d1 == 2 && n1 < v) || d2 == 1 || (d2 == 2 && n2 < v);
                 if (!skip) {
                     if (adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
@@ -538,7 +583,7 @@ __device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st, uin
             uint32_t v = base + lane_id();
             uint32_t cand = kNone;
             if (v < w.n) {
-                int dv = w.deg[v];
+                int dv = w.deg(v);
                 lanes_add(st, dv, v);
                 if (dv > 0 && (uint32_t)dv + d + total >= T) cand = v;
             }
@@ -603,21 +648,18 @@ template <class N>
 __device__ void take_vertex(const N &w, uint32_t v, uint32_t &nc)
 {
     w.sync();
-    if (lane_id() == 0) w.deg[v] = -1;
+    if (lane_id() == 0) w.remove(v);
     w.sync();
-    using S2 = typename N::s2_t;
     const uint32_t b = w.src[v], e = w.src[v + 1];
     for (uint32_t x0 = b; x0 < e; x0 += kWave) {
         const uint32_t x = x0 + lane_id();
         uint32_t leaf = kNone;
         if (x < e) {
             const uint32_t t = w.dst[x];
-            const int dt = w.deg[t];
+            const int dt = w.deg(t);
             if (dt != -1) {
                 // the neighbours of one vertex are distinct: plain updates
-                w.deg[t] = (typename N::deg_t)(dt - 1);
-                w.s1[t] -= v;
-                w.s2[t] -= (S2)v * (S2)v;
+                w.drop_plain(t, v, dt);
                 if (dt == 2) leaf = t;
             }
         }
@@ -642,7 +684,7 @@ __device__ uint32_t take_neighbours(const N &w, uint32_t v, uint32_t &nc)
             uint32_t cand = kNone;
             if (x < e) {
                 uint32_t t = w.dst[x];
-                if (w.deg[t] != -1) cand = t;
+                if (w.deg(t) != -1) cand = t;
             }
             L = append(w, L, cand, false);
         }
@@ -662,19 +704,17 @@ __device__ void init_root(const N &w, const int32_t *deg0, uint32_t npad)
     for (uint32_t v = lane_id(); v < npad; v += kWave) {
         uint32_t s = 0;
         S2 q = 0;
+        int dv = -1;
         if (v < w.n) {
-            w.deg[v] = (typename N::deg_t)deg0[v];
+            dv = deg0[v];
             const uint32_t e = w.src[v + 1];
             for (uint32_t x = w.src[v]; x < e; ++x) {
                 const uint32_t u = w.dst[x];
                 s += u;
                 q += (S2)u * (S2)u;
             }
-        } else {
-            w.deg[v] = (typename N::deg_t)-1;
         }
-        w.s1[v] = s;
-        w.s2[v] = q;
+        w.init_vertex(v, dv, s, q);
     }
     w.sync();
 }
@@ -775,7 +815,7 @@ __device__ bool try_dequeue(const SolveArgs &a, const N &w, uint32_t &g, uint32_
     slot = uni(slot);
     ticket = uni(ticket);
     fence_acq();
-    copy_in<false>(w.deg, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, slot_bytes);
+    copy_in<false>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, slot_bytes);
     w.sync();
     uint4 meta = a.q_meta[slot];
     g = uni(meta.x);
@@ -800,7 +840,7 @@ __device__ void write_cover(const SolveArgs &a, const N &w, const GDesc &gd)
     uint32_t *cw = a.covers + gd.off_cover;
     for (uint32_t base = 0; base < w.n; base += kWave) {
         uint32_t v = base + lane_id();
-        uint64_t m = __ballot(v < w.n && w.deg[v] == -1);
+        uint64_t m = __ballot(v < w.n && w.deg(v) == -1);
         if (lane_id() == 0) {
             cw[base / 32] = (uint32_t)m;
             cw[base / 32 + 1] = (uint32_t)(m >> 32);
@@ -1070,19 +1110,29 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
     const uint32_t wid = blockIdx.x * (blockDim.x / kWave) + wib;
     if (wid >= a.workers) return;
 
-    // node storage: deg [npad DegT] | s1 [npad u32] | s2 [npad] | list [lcap u32] | mark [mwords u32] |
-    // xs [192 i32] | cand [npad]; a stack or worklist slot holds deg | s1 | s2
+    // node storage: node state (npad * node_bytes_per_vertex) | list [lcap u32] | mark [mwords u32] |
+    // xs [192 i32] | cand [npad]; a stack or worklist slot holds the node state
     char *store = HBM ? a.hbm_nodes + (size_t)wid * a.worker_bytes
                       : reinterpret_cast<char *>(smem4) + (size_t)wib * a.worker_bytes;
     const uint32_t slot_bytes = a.npad * node_bytes_per_vertex<DegT>();
-    Node<DegT, HBM> w;
-    w.deg = reinterpret_cast<DegT *>(store);
-    w.s1 = reinterpret_cast<uint32_t *>(store + (size_t)a.npad * sizeof(DegT));
-    w.s2 = reinterpret_cast<typename Node<DegT, HBM>::s2_t *>(store + (size_t)a.npad * (sizeof(DegT) + 4u));
+    using NodeT = Node<DegT, HBM>;
+    NodeT w;
+    w.base = store;
+    if constexpr (NodeT::kPacked) {
+        w.dw = reinterpret_cast<uint32_t *>(store);
+        w.deg32 = nullptr;
+        w.s1w = nullptr;
+        w.s2 = reinterpret_cast<typename NodeT::s2_t *>(store + (size_t)a.npad * 4u);
+    } else {
+        w.dw = nullptr;
+        w.deg32 = reinterpret_cast<int32_t *>(store);
+        w.s1w = reinterpret_cast<uint32_t *>(store + (size_t)a.npad * 4u);
+        w.s2 = reinterpret_cast<typename NodeT::s2_t *>(store + (size_t)a.npad * 8u);
+    }
     w.list = reinterpret_cast<uint32_t *>(store + slot_bytes);
     w.mark = w.list + a.lcap;
     w.xs = reinterpret_cast<int *>(w.mark + a.mwords);
-    w.cand = reinterpret_cast<typename Node<DegT, HBM>::cand_t *>(w.xs + 3 * kWave);
+    w.cand = reinterpret_cast<typename NodeT::cand_t *>(w.xs + 3 * kWave);
     w.ncap = a.npad;
     w.src = nullptr;
     w.dst = nullptr;
@@ -1116,7 +1166,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                 // pop (popStack, src/stack.cuh:10-20), then build the stored
                 // right child
                 const uint32_t ts = (uint32_t)top % a.depth;
-                copy_in<HBM>(w.deg, stk + (size_t)ts * slot_bytes, slot_bytes);
+                copy_in<HBM>(w.base, stk + (size_t)ts * slot_bytes, slot_bytes);
                 uint2 m = stk_meta[ts];
                 d = uni(m.x);
                 uint32_t v = uni(m.y);
@@ -1260,7 +1310,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
         }
         ++top;
         const uint32_t ts = (uint32_t)top % a.depth;
-        copy_out<HBM>(stk + (size_t)ts * slot_bytes, w.deg, slot_bytes);
+        copy_out<HBM>(stk + (size_t)ts * slot_bytes, w.base, slot_bytes);
         if (lane_id() == 0) stk_meta[ts] = make_uint2(d, v);
         if (a.version == 1) {
             // Donate while workers wait and the worklist holds fewer items
