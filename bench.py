@@ -250,15 +250,15 @@ def served(ctx, args, out):
             reqs.append((gi, vcgpu.Job(vcgpu.VC_PVC, k, ubs[gi][0], 0), ("pvc", int(k >= opt)), "small"))
     gaps = rng.exponential(1.0 / args.rate, size=n_req)
     # two servers: the small class takes most of each CU, the large class the rest
-    # The small server leaves one wave slot per SIMD free (5 of its 6
-    # resident workgroups per CU) so that the large server's waves are
-    # resident beside it.  The large server's stacks hold a full path of
-    # any large request (admission rejects requests needing more).
+    # The large server's stacks hold a full path of any large request
+    # (admission rejects requests needing more).
     large_depth = max(min(pool[gi].n, ubs[gi][0] + 1) + 1 for gi in large_ids)
-    small_srv = vcgpu.Server(ctx, max_vertices=512, slots=4096, arena_bytes=256 << 20,
-                             num_blocks=args.small_blocks, time_limit_ms=args.time_limit_ms)
+    # The large server starts first: its waves need more registers, and a
+    # persistent kernel launched second only gets the CUs' leftovers.
     large_srv = vcgpu.Server(ctx, max_vertices=20000, slots=64, arena_bytes=256 << 20, block_dim=64,
                              num_blocks=args.large_blocks, stack_depth=large_depth, time_limit_ms=args.time_limit_ms)
+    small_srv = vcgpu.Server(ctx, max_vertices=512, slots=4096, arena_bytes=256 << 20,
+                             num_blocks=args.small_blocks, time_limit_ms=args.time_limit_ms)
     arrivals = [0] * n_req
     tickets = [None] * n_req
     results = [None] * n_req
@@ -345,11 +345,12 @@ def main():
                     choices=["batch", "single_er", "single_reg3", "single_ba", "served"])
     ap.add_argument("--requests", type=int, default=2000, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=1000.0, help="served: Poisson arrival rate (requests/s)")
-    ap.add_argument("--small-blocks", type=int, default=256 * 5, help="served: workgroups of the small-graph server")
-    ap.add_argument("--large-blocks", type=int, default=16,
+    ap.add_argument("--small-blocks", type=int, default=0,
+                    help="served: workgroups of the small-graph server (0 = occupancy x CUs)")
+    ap.add_argument("--large-blocks", type=int, default=32,
                     help="served: one-wave workgroups of the large-graph server (each holds an exact-depth stack, "
                          "about 2 GB for BA n=20000)")
-    ap.add_argument("--large-budget", type=int, default=1 << 16, help="served: node budget of a large request")
+    ap.add_argument("--large-budget", type=int, default=1 << 12, help="served: node budget of a large request")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--base-seed", type=int, default=1000)
     ap.add_argument("--max-nodes", type=int, default=1 << 20, help="per-graph GPU tree-node budget (batch)")

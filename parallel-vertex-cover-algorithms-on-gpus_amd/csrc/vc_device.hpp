@@ -66,7 +66,7 @@ struct Global {
     uint32_t error;           // nonzero: an internal bound was hit (see kErr*)
     uint32_t cancel;          // cancel request (host, or the deadline)
     uint32_t timed_out;       // the deadline passed before the search finished
-    uint64_t t0;              // wall clock at init (device clock ticks)
+    uint64_t t0;              // wall clock when the first solve worker started (0 = none yet)
     uint32_t pad5[10];
     // serving: device copy of the host's request tail / shutdown flag, and the
     // time of the last host poll (polls are rate-limited to one per interval)
@@ -149,6 +149,11 @@ struct SolveArgs {
     HostCtrl *host;            // pinned
     Completion *comp;          // pinned: one record per slot
     uint64_t poll_ticks;       // minimum interval between host polls
+    // longest sleep of an idle worker between polls, as log2 of 512-cycle
+    // sleeps: every poll reads launch-wide words with device scope, so
+    // thousands of idle waves polling often load the memory channels that
+    // the busy waves (and other kernels) use
+    uint32_t idle_cap;
 };
 
 }  // namespace vcg

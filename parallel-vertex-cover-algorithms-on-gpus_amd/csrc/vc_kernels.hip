@@ -154,9 +154,10 @@ __device__ __forceinline__ T fetch_add(T *p, T v)
 __device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 __device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
-__device__ __forceinline__ void backoff(uint32_t step)
+// Sleeps 2^min(step, cap) x 512 cycles.
+__device__ __forceinline__ void backoff(uint32_t step, uint32_t cap = 6u)
 {
-    uint32_t reps = 1u << (step < 6 ? step : 6);
+    uint32_t reps = 1u << (step < cap ? step : cap);
     for (uint32_t i = 0; i < reps; ++i) __builtin_amdgcn_s_sleep(8);
 }
 
@@ -719,21 +720,35 @@ __device__ void init_root(const N &w, const int32_t *deg0, uint32_t npad)
     w.sync();
 }
 
-// Node storage <-> HBM slot copies (bytes % 16 == 0, both sides 16-aligned).
-template <bool HBM>
-__device__ __forceinline__ void copy_out(void *__restrict__ g, const void *s, uint32_t bytes)
+// Copies the node state of the first cnt vertices between two node-state
+// layouts with the same npad (node storage <-> a stack or worklist slot).
+// A slot is sized for the largest graph of the launch; only the current
+// graph's vertices travel (rounded up to whole 16-byte vectors), and all
+// loads of a lane are issued before its stores.
+template <class N>
+__device__ __forceinline__ void copy_state(char *__restrict__ dst, const char *__restrict__ src, uint32_t npad,
+                                           uint32_t cnt)
 {
-    const int4 *s4 = reinterpret_cast<const int4 *>(s);
-    int4 *g4 = reinterpret_cast<int4 *>(g);
-    for (uint32_t i = lane_id(); i < bytes / 16; i += kWave) g4[i] = s4[i];
-}
-template <bool HBM>
-__device__ __forceinline__ void copy_in(void *s, const void *__restrict__ g, uint32_t bytes)
-{
-    const int4 *g4 = reinterpret_cast<const int4 *>(g);
-    int4 *s4 = reinterpret_cast<int4 *>(s);
-    for (uint32_t i = lane_id(); i < bytes / 16; i += kWave) s4[i] = g4[i];
-    nsync<HBM>();
+    const uint32_t q = (cnt + 3u) >> 2;  // 16-byte vectors per array of 4-byte entries
+    const int4 *s4 = reinterpret_cast<const int4 *>(src);
+    int4 *d4 = reinterpret_cast<int4 *>(dst);
+    const uint32_t o1 = npad >> 2;       // second array, at npad * 4 bytes
+    if constexpr (N::kPacked) {
+        for (uint32_t i = lane_id(); i < q; i += kWave) {
+            const int4 x = s4[i], y = s4[o1 + i];
+            d4[i] = x;
+            d4[o1 + i] = y;
+        }
+    } else {
+        const uint32_t o2 = npad >> 1;   // 8-byte s2 entries, at npad * 8 bytes
+        for (uint32_t i = lane_id(); i < q; i += kWave) {
+            const int4 x = s4[i], y = s4[o1 + i], z0 = s4[o2 + 2u * i], z1 = s4[o2 + 2u * i + 1u];
+            d4[i] = x;
+            d4[o1 + i] = y;
+            d4[o2 + 2u * i] = z0;
+            d4[o2 + 2u * i + 1u] = z1;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -743,7 +758,8 @@ __device__ __forceinline__ void copy_in(void *s, const void *__restrict__ g, uin
 // Writes one subtree (degrees at src, cover size d, pending branch vertex v
 // of graph g) into the ring if the occupancy guard admits it.  The caller
 // has already decided that a donation is wanted.
-__device__ bool try_enqueue(const SolveArgs &a, const void *src, uint32_t g, uint32_t d, uint32_t v,
+template <class N>
+__device__ bool try_enqueue(const SolveArgs &a, const N &w, const char *src, uint32_t g, uint32_t d, uint32_t v,
                             uint32_t slot_bytes)
 {
     uint32_t ok = 0, slot = 0, ticket = 0;
@@ -774,7 +790,7 @@ __device__ bool try_enqueue(const SolveArgs &a, const void *src, uint32_t g, uin
     slot = uni(slot);
     ticket = uni(ticket);
     fence_acq();
-    copy_out<false>(static_cast<char *>(a.q_deg) + (size_t)slot * slot_bytes, src, slot_bytes);
+    copy_state<N>(static_cast<char *>(a.q_deg) + (size_t)slot * slot_bytes, src, a.npad, w.n);
     if (lane_id() == 0) a.q_meta[slot] = make_uint4(g, d, v, 0u);
     fence_rel();
     if (lane_id() == 0) {
@@ -815,12 +831,13 @@ __device__ bool try_dequeue(const SolveArgs &a, const N &w, uint32_t &g, uint32_
     slot = uni(slot);
     ticket = uni(ticket);
     fence_acq();
-    copy_in<false>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, slot_bytes);
-    w.sync();
     uint4 meta = a.q_meta[slot];
     g = uni(meta.x);
     d = uni(meta.y);
     v = uni(meta.z);
+    copy_state<N>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, a.npad,
+                  uni(a.gdesc[g].n));
+    w.sync();
     fence_rel();
     if (lane_id() == 0) {
         st_rel(&a.tickets[slot], ticket + 1u);
@@ -1090,7 +1107,7 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
         uint32_t late = 0;
         if (lane_id() == 0) late = past_deadline(a);
         if (uni(late)) return r;
-        backoff(++spins);
+        backoff(++spins, a.idle_cap);
     }
 }
 
@@ -1144,7 +1161,14 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
     char *stk = static_cast<char *>(a.stk) + (size_t)wid * a.depth * slot_bytes;
     uint2 *stk_meta = a.stk_meta + (size_t)wid * a.depth;
 
-    if (lane_id() == 0) fetch_add(&a.glob->counter, kRegOne);
+    if (lane_id() == 0) {
+        fetch_add(&a.glob->counter, kRegOne);
+        // the time limit counts from the first worker's start, not from the
+        // launch: a persistent kernel can wait for CUs held by another one
+        uint64_t zero = 0;
+        __hip_atomic_compare_exchange_strong(&a.glob->t0, &zero, (uint64_t)wall_clock64(),
+                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 
     uint32_t g = kNone;
     GDesc gd = {};
@@ -1166,7 +1190,8 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                 // pop (popStack, src/stack.cuh:10-20), then build the stored
                 // right child
                 const uint32_t ts = (uint32_t)top % a.depth;
-                copy_in<HBM>(w.base, stk + (size_t)ts * slot_bytes, slot_bytes);
+                copy_state<NodeT>(w.base, stk + (size_t)ts * slot_bytes, a.npad, w.n);
+                w.sync();
                 uint2 m = stk_meta[ts];
                 d = uni(m.x);
                 uint32_t v = uni(m.y);
@@ -1287,7 +1312,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                 for (uint32_t spins = 0;; ++spins) {
                     const uint32_t bs = (uint32_t)bot % a.depth;
                     uint2 m = stk_meta[bs];
-                    if (try_enqueue(a, stk + (size_t)bs * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
+                    if (try_enqueue(a, w, stk + (size_t)bs * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
                         ++bot;
                         ++donations;
                         moved = true;
@@ -1310,7 +1335,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
         }
         ++top;
         const uint32_t ts = (uint32_t)top % a.depth;
-        copy_out<HBM>(stk + (size_t)ts * slot_bytes, w.base, slot_bytes);
+        copy_state<NodeT>(stk + (size_t)ts * slot_bytes, w.base, a.npad, w.n);
         if (lane_id() == 0) stk_meta[ts] = make_uint2(d, v);
         if (a.version == 1) {
             // Donate while workers wait and the worklist holds fewer items
@@ -1324,7 +1349,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
             if (waiting > 0 && enq < waiting && enq < (int)a.qthreshold && top >= bot) {
                 const uint32_t bs = (uint32_t)bot % a.depth;
                 uint2 m = stk_meta[bs];
-                if (try_enqueue(a, stk + (size_t)bs * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
+                if (try_enqueue(a, w, stk + (size_t)bs * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
                     ++bot;
                     ++donations;
                 }
@@ -1360,8 +1385,7 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
     for (uint32_t i = tid; i < a.qsize; i += stride) a.tickets[i] = 0u;
     for (uint32_t i = tid; i < a.workers; i += stride) a.wstats[i] = WorkerStats{};
     if (tid == 0) {
-        Global z = {};
-        z.t0 = wall_clock64();
+        Global z = {};  // t0 = 0: the first solve worker to start sets it
         *a.glob = z;
     }
 }
@@ -1384,6 +1408,8 @@ template __global__ void solve_kernel<int32_t, true, true>(SolveArgs);
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 #include "vcgpu.h"
@@ -1396,7 +1422,7 @@ using vcg::SolveArgs;
 struct vc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // solve start / end / results on host
     int cu_count = 0;
     int wall_khz = 100000;  // device wall-clock rate
     size_t total_mem = 0;
@@ -1406,7 +1432,47 @@ struct vc_ctx {
     size_t scratch_bytes = 0;
     // autotuner cache: (vertex-count bucket, density bucket) -> chosen params
     std::map<std::pair<int, int>, vc_params> tuned;
+    // hipFree / hipHostFree wait for the whole device, including a running
+    // server's persistent kernel; while a server runs, frees are deferred
+    // until the last one stops (or the context is destroyed).
+    std::mutex mem_mu;
+    int active_servers = 0;
+    std::vector<std::pair<void *, bool>> deferred;  // (pointer, pinned host memory)
 };
+
+void free_now(void *p, bool host)
+{
+    if (host)
+        (void)hipHostFree(p);
+    else
+        (void)hipFree(p);
+}
+
+// Frees p now, or later if a server of this context is running.
+void ctx_release(vc_ctx *ctx, void *p, bool host)
+{
+    if (!p) return;
+    if (ctx) {
+        std::lock_guard<std::mutex> lk(ctx->mem_mu);
+        if (ctx->active_servers > 0) {
+            ctx->deferred.emplace_back(p, host);
+            return;
+        }
+    }
+    free_now(p, host);
+}
+
+// Releases the deferred frees once no server runs.
+void ctx_flush_deferred(vc_ctx *ctx)
+{
+    std::vector<std::pair<void *, bool>> d;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mem_mu);
+        if (ctx->active_servers > 0) return;
+        d.swap(ctx->deferred);
+    }
+    for (auto &x : d) free_now(x.first, x.second);
+}
 
 // What a captured dispatch depends on; a different value means re-capture.
 struct DispatchKey {
@@ -1648,6 +1714,7 @@ void fill_args(SolveArgs &a, const Plan &pl, char *base, uint32_t version, uint3
     a.roots = pl.roots;
     a.limit_ticks = limit_ticks;
     a.hbm_nodes = pl.variant != kLds16 ? base + pl.off_hbm : nullptr;
+    a.idle_cap = 6u;  // about 14 us between polls of an idle worker
 }
 
 template <bool SERVE>
@@ -1718,7 +1785,7 @@ vc_status vc_ctx_create(int device, vc_ctx **out)
     int khz = 0;
     if (hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device)) && khz > 0) c->wall_khz = khz;
     if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) || !hip_ok(hipEventCreate(&c->ev0)) ||
-        !hip_ok(hipEventCreate(&c->ev1))) {
+        !hip_ok(hipEventCreate(&c->ev1)) || !hip_ok(hipEventCreate(&c->ev2))) {
         delete c;
         return VC_EDEVICE;
     }
@@ -1731,8 +1798,11 @@ vc_status vc_ctx_destroy(vc_ctx *c)
     if (!c) return VC_EINVAL;
     (void)hipSetDevice(c->device);
     if (c->scratch) (void)hipFree(c->scratch);
+    for (auto &x : c->deferred) free_now(x.first, x.second);
+    c->deferred.clear();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return VC_OK;
@@ -1809,14 +1879,14 @@ vc_status vc_graphs_free(vc_ctx *ctx, vc_graphs *s)
 {
     if (!s) return VC_EINVAL;
     if (ctx) (void)hipSetDevice(ctx->device);
-    if (s->d_words) (void)hipFree(s->d_words);
-    if (s->d_desc) (void)hipFree(s->d_desc);
-    if (s->d_covers) (void)hipFree(s->d_covers);
-    if (s->d_state) (void)hipFree(s->d_state);
-    if (s->h_state) (void)hipHostFree(s->h_state);
-    if (s->h_glob) (void)hipHostFree(s->h_glob);
-    if (s->h_ws) (void)hipHostFree(s->h_ws);
-    if (s->h_cov) (void)hipHostFree(s->h_cov);
+    ctx_release(ctx, s->d_words, false);
+    ctx_release(ctx, s->d_desc, false);
+    ctx_release(ctx, s->d_covers, false);
+    ctx_release(ctx, s->d_state, false);
+    ctx_release(ctx, s->h_state, true);
+    ctx_release(ctx, s->h_glob, true);
+    ctx_release(ctx, s->h_ws, true);
+    ctx_release(ctx, s->h_cov, true);
     if (s->exec) (void)hipGraphExecDestroy(s->exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
     delete s;
@@ -1874,7 +1944,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         return VC_EDEVICE;
 
     if (ctx->scratch_bytes < pl.total) {
-        if (ctx->scratch) (void)hipFree(ctx->scratch);
+        ctx_release(ctx, ctx->scratch, false);
         ctx->scratch = nullptr;
         ctx->scratch_bytes = 0;
         if (!hip_ok(hipMalloc(&ctx->scratch, pl.total))) return VC_ENOMEM;
@@ -1904,7 +1974,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
              hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_glob), sizeof(Global))) &&
              hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_cov), sizeof(uint32_t) * set->cover_words));
     if (ok && set->ws_cap < pl.workers) {
-        if (set->h_ws) (void)hipHostFree(set->h_ws);
+        ctx_release(ctx, set->h_ws, true);
         set->h_ws = nullptr;
         ok = hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_ws), sizeof(vcg::WorkerStats) * pl.workers));
         set->ws_cap = ok ? pl.workers : 0;
@@ -1973,7 +2043,10 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     } else {
         if (!enqueue_dispatch(true)) return VC_EDEVICE;
     }
-    if (!hip_ok(hipStreamSynchronize(ctx->stream))) return VC_EDEVICE;
+    // Wait for this solve's own completion event, not for the stream: when
+    // two streams share a hardware queue, a stream-level wait can also wait
+    // for another stream's persistent kernel (a running server).
+    if (!hip_ok(hipEventRecord(ctx->ev2, ctx->stream)) || !hip_ok(hipEventSynchronize(ctx->ev2))) return VC_EDEVICE;
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) ms = 0.f;
     const GState *hs = set->h_state;
@@ -2204,6 +2277,7 @@ constexpr uint64_t kUnit = 64;
 
 struct vc_server {
     vc_ctx *ctx = nullptr;
+    bool counted = false;  // included in ctx->active_servers
     vc_server_config cfg{};
     Plan pl{};
     SolveArgs a{};
@@ -2345,7 +2419,7 @@ void server_loop(vc_server *s)
         // a cheap liveness check every few idle rounds (a stream query is not
         // a copy and needs no CU)
         bool alive = true;
-        if (!work && ++idle_rounds % 64 == 0) alive = hipStreamQuery(s->kstream) == hipErrorNotReady;
+        if (!work && ++idle_rounds % 64 == 0) alive = hipEventQuery(s->ev1) == hipErrorNotReady;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             if (!alive) s->dead = true;
@@ -2353,9 +2427,10 @@ void server_loop(vc_server *s)
         }
         if (!work) std::this_thread::sleep_for(std::chrono::microseconds(10));
     }
-    // let the kernel drain and exit
+    // let the kernel drain and exit (its own end event: a stream-level wait
+    // could also wait for another server's kernel on a shared hardware queue)
     __atomic_store_n(&s->host->shutdown, 1u, __ATOMIC_RELEASE);
-    (void)hipStreamSynchronize(s->kstream);
+    (void)hipEventSynchronize(s->ev1);
     server_collect(s);
     std::lock_guard<std::mutex> lk(s->mu);
     auto fail = [&](uint64_t ticket) {
@@ -2375,19 +2450,25 @@ void server_loop(vc_server *s)
 
 void server_free(vc_server *s)
 {
-    if (s->scratch) (void)hipFree(s->scratch);
-    if (s->d_desc) (void)hipFree(s->d_desc);
-    if (s->d_state) (void)hipFree(s->d_state);
-    if (s->d_arena) (void)hipFree(s->d_arena);
-    if (s->host) (void)hipHostFree(s->host);
-    if (s->comp) (void)hipHostFree(s->comp);
-    if (s->h_desc) (void)hipHostFree(s->h_desc);
-    if (s->h_ring) (void)hipHostFree(s->h_ring);
-    if (s->h_arena) (void)hipHostFree(s->h_arena);
+    vc_ctx *ctx = s->ctx;
+    if (s->counted) {
+        std::lock_guard<std::mutex> lk(ctx->mem_mu);
+        --ctx->active_servers;
+    }
+    ctx_release(ctx, s->scratch, false);
+    ctx_release(ctx, s->d_desc, false);
+    ctx_release(ctx, s->d_state, false);
+    ctx_release(ctx, s->d_arena, false);
+    ctx_release(ctx, s->host, true);
+    ctx_release(ctx, s->comp, true);
+    ctx_release(ctx, s->h_desc, true);
+    ctx_release(ctx, s->h_ring, true);
+    ctx_release(ctx, s->h_arena, true);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->kstream) (void)hipStreamDestroy(s->kstream);
     delete s;
+    ctx_flush_deferred(ctx);
 }
 
 template <class T>
@@ -2410,10 +2491,15 @@ vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **
     vc_server *s = new vc_server();
     s->ctx = ctx;
     s->cfg = *cfg;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mem_mu);
+        ++ctx->active_servers;
+        s->counted = true;
+    }
     const uint32_t slots = cfg->slots ? cfg->slots : 4096u;
     const uint64_t arena_bytes = cfg->arena_bytes ? cfg->arena_bytes : (1ull << 30);
     if (arena_bytes / 4 >= 0xFFFFFFF0ull) {
-        delete s;
+        server_free(s);
         return VC_EINVAL;
     }
     vc_params p;
@@ -2425,7 +2511,7 @@ vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **
     const uint32_t depth = cfg->stack_depth ? cfg->stack_depth : cfg->max_vertices + 1u;
     vc_status st = plan_launch(ctx, cfg->max_vertices, depth, &p, s->pl, true);
     if (st != VC_OK) {
-        delete s;
+        server_free(s);
         return st;
     }
     s->ring_size = next_pow2(std::max<uint32_t>(2u * slots, 1024u));
@@ -2464,6 +2550,10 @@ vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **
     a.host = s->host;
     a.comp = s->comp;
     a.poll_ticks = std::max<uint64_t>(1, (uint64_t)ctx->wall_khz * 2u / 1000u);  // about 2 microseconds
+    // A server is idle most of the time: its idle waves poll about every
+    // 220 us, and with thousands of them polling at random phases a new
+    // request is still seen within the 2 us host-poll interval.
+    a.idle_cap = 10u;
     hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, s->kstream, a, 1u);
     ok = hip_ok(hipGetLastError()) && hip_ok(hipEventRecord(s->ev0, s->kstream)) &&
          launch_solve(s->pl, s->kstream, a) && hip_ok(hipEventRecord(s->ev1, s->kstream));
@@ -2552,13 +2642,19 @@ vc_status vc_server_stop(vc_server *s, vc_launch_info *info)
         s->stopping = true;
     }
     if (s->th.joinable()) s->th.join();
-    vc_status st = hip_ok(hipStreamSynchronize(s->kstream)) ? VC_OK : VC_EDEVICE;
+    vc_status st = hip_ok(hipEventSynchronize(s->ev1)) ? VC_OK : VC_EDEVICE;
     std::vector<vcg::WorkerStats> wn(s->pl.workers);
     Global hg;
     std::memset(&hg, 0, sizeof(hg));
-    bool got = hip_ok(hipMemcpy(wn.data(), s->a.wstats, sizeof(vcg::WorkerStats) * s->pl.workers,
-                                hipMemcpyDeviceToHost)) &&
-               hip_ok(hipMemcpy(&hg, s->a.glob, sizeof(Global), hipMemcpyDeviceToHost));
+    // statistics through this server's stream, waited for by an event (see
+    // server_loop)
+    hipEvent_t done = nullptr;
+    bool got = hip_ok(hipEventCreateWithFlags(&done, hipEventDisableTiming)) &&
+               hip_ok(hipMemcpyAsync(wn.data(), s->a.wstats, sizeof(vcg::WorkerStats) * s->pl.workers,
+                                     hipMemcpyDeviceToHost, s->kstream)) &&
+               hip_ok(hipMemcpyAsync(&hg, s->a.glob, sizeof(Global), hipMemcpyDeviceToHost, s->kstream)) &&
+               hip_ok(hipEventRecord(done, s->kstream)) && hip_ok(hipEventSynchronize(done));
+    if (done) (void)hipEventDestroy(done);
     if (!got) st = VC_EDEVICE;
     if (hg.error) st = VC_EDEVICE;
     if (info) {

@@ -121,3 +121,47 @@ def test_server_admits_by_stack_footprint(ctx):
         assert out is not None and out[0].solved and out[0].minimum == O.mvc(O.from_edges(n2, e2))[0]
     finally:
         srv.stop()
+
+
+def test_hbm_server_beside_a_full_small_server(ctx):
+    """Two persistent servers share the GPU: one whose node storage is in
+    HBM (max_vertices 20000 does not fit LDS) and a small-graph server with
+    the occupancy-derived grid.  The larger-footprint server starts first
+    (a persistent kernel launched second only gets the CUs left over).  Both
+    answer exactly while both run."""
+    big = []
+    for s in range(2):
+        n, e = 3000, graphgen.barabasi_albert(3000, 3, 900 + s)
+        og = O.from_edges(n, e)
+        opt, _, _ = O.mvc(og)
+        g = vcgpu.csr_from_edges(n, e)
+        U, _ = vcgpu.upper_bound(g, seed=9)
+        big.append((g, og, U, opt))
+    depth = max(min(g.n, U + 1) + 1 for g, _, U, _ in big)  # PVC at k = opt = U needs U + 2
+    large = vcgpu.Server(ctx, max_vertices=20000, slots=8, block_dim=64, num_blocks=8, stack_depth=depth,
+                         time_limit_ms=60000)
+    small = vcgpu.Server(ctx, max_vertices=512, slots=64, time_limit_ms=60000)
+    try:
+        reqs = _requests()[:16]
+        ts = [small.submit(g, job) for g, _, job, _, _ in reqs]
+        tl = [(large.submit(g, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0)),
+               large.submit(g, vcgpu.Job(vcgpu.VC_PVC, opt - 1, U, 0)),
+               large.submit(g, vcgpu.Job(vcgpu.VC_PVC, opt, U, 0))) for g, _, U, opt in big]
+        for t, (g, og, job, opt, ucov) in zip(ts, reqs):
+            out = small.wait(t, 60000)
+            assert out is not None
+            _check(out[0], og, job, opt, ucov)
+        for (tm, tno, tyes), (g, og, U, opt) in zip(tl, big):
+            outs = [large.wait(t, 60000) for t in (tm, tno, tyes)]
+            assert all(o is not None for o in outs), "large request did not finish"
+            rm, rno, ryes = (o[0] for o in outs)
+            assert rm.solved and rm.minimum == opt
+            if rm.cover is not None:
+                assert O.cover_is_valid(og, np.asarray(rm.cover, np.uint8)) and int(np.sum(rm.cover)) == opt
+            assert rno.solved and rno.k_found == 0
+            assert ryes.solved and ryes.k_found == 1 and ryes.cover is not None
+            assert O.cover_is_valid(og, np.asarray(ryes.cover, np.uint8)) and int(np.sum(ryes.cover)) <= opt
+    finally:
+        info_l = large.stop()
+        small.stop()
+    assert info_l.use_global_memory == 1
