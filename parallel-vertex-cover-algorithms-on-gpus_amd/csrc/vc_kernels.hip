@@ -302,6 +302,8 @@ struct Node {
 template <class N>
 __device__ __forceinline__ uint32_t append(const N &w, uint32_t L, uint32_t cand, bool dedupe)
 {
+    // most scan chunks mark nothing: leave at once (a wave-uniform branch)
+    if (__ballot(cand != kNone) == 0ull) return L;
     bool keep = cand != kNone;
     uint32_t bit = 1u << (cand & 31u);
     if (dedupe && keep) {
@@ -349,15 +351,19 @@ __device__ __forceinline__ unsigned long long isqrt_round(unsigned long long x)
     return (unsigned long long)(__builtin_sqrt((double)x) + 0.5);
 }
 
-// Top bit of s2 on a degree-2 vertex: its two neighbours were found not to
-// be adjacent.  While deg(v) == 2 its neighbour pair cannot change, and
-// degrees never grow along a path of the tree, so the result holds in every
-// descendant node; once deg(v) < 2 the bit is never read.  Below the bit,
-// n1^2 + n2^2 < 2^31 (int16 degrees) or 2^63, so it never carries.
-template <typename S2> constexpr S2 kNotTriangle = (S2)1 << (8 * sizeof(S2) - 1);
+// Top bit of s2 on a degree-2 vertex: the triangle rule examined it and it
+// did not fire, i.e. a neighbour had degree 1 or degree 2 and a lower index
+// (the skip), or its two neighbours are not adjacent.  Both outcomes are
+// final while deg(v) == 2: the neighbour pair cannot change, degrees never
+// grow along a path of the tree, a neighbour of degree 1 keeps v as its one
+// neighbour, and a degree-2 neighbour can only fall to 1.  So the bit holds
+// in every descendant node, and a settled vertex is not decoded again; once
+// deg(v) < 2 the bit is never read.  Below the bit, n1^2 + n2^2 < 2^31
+// (int16 degrees) or 2^63, so it never carries.
+template <typename S2> constexpr S2 kSettled = (S2)1 << (8 * sizeof(S2) - 1);
 
 // The two live neighbours n1 < n2 of a degree-2 vertex from its sketch
-// (s2 without the kNotTriangle bit).  False when the sketch does not decode
+// (s2 without the kSettled bit).  False when the sketch does not decode
 // to two distinct vertices below n (impossible for a consistent node).
 template <class N>
 __device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, typename N::s2_t q, uint32_t &n1,
@@ -547,18 +553,16 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
             uint32_t n1, n2;
             S2 q = 0;
             if (dv == 2) q = w.s2[v];
-            if (dv == 2 && (q & kNotTriangle<S2>) == 0 && two_neighbours(w, v, q, n1, n2)) {
+            if (dv == 2 && (q & kSettled<S2>) == 0 && two_neighbours(w, v, q, n1, n2)) {
                 // skipped when a neighbour has degree 1, or degree 2 and a
                 // lower index (src/helperFunctions.cuh:379-385)
                 const int d1 = w.deg(n1), d2 = w.deg(n2);
                 const bool skip = d1 == 1 || (d1 == 2 && n1 < v) || d2 == 1 || (d2 == 2 && n2 < v);
-                if (!skip) {
-                    if (adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
-                        c1 = n1;
-                        c2 = n2;
-                    } else {
-                        w.s2[v] = q | kNotTriangle<S2>;
-                    }
+                if (!skip && adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
+                    c1 = n1;
+                    c2 = n2;
+                } else {
+                    w.s2[v] = q | kSettled<S2>;
                 }
             }
             L = append(w, L, c1, true);
