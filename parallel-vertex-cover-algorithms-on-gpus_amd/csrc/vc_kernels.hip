@@ -318,7 +318,7 @@ __device__ __forceinline__ uint32_t append(const N &w, uint32_t L, uint32_t cand
         else if (dedupe)
             __hip_atomic_fetch_and(&w.mark[cand >> 5], ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
-    return min(L + (uint32_t)__popcll(m), w.lcap);
+    return uni(min(L + (uint32_t)__popcll(m), w.lcap));
 }
 
 // Leaf candidates: every vertex whose degree falls from 2 to 1 is appended
@@ -334,7 +334,7 @@ __device__ __forceinline__ uint32_t add_leaf_cands(const N &w, uint32_t nc, uint
         const uint32_t pos = nc + lanes_below(m);
         if (pos < w.ncap) w.cand[pos] = (typename N::cand_t)t;
     }
-    return nc + (uint32_t)__popcll(m);
+    return uni(nc + (uint32_t)__popcll(m));
 }
 
 // Live vertex t loses its live neighbour u (several lanes may hit the same t).
@@ -389,6 +389,7 @@ __device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, typename 
 template <class N>
 __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
 {
+    L = uni(L);
     w.sync();
     for (uint32_t i = lane_id(); i < L; i += kWave) w.remove(w.list[i]);
     w.sync();
@@ -1117,6 +1118,26 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
 
 __device__ __forceinline__ long long sq64(int x) { return (long long)x * (long long)x; }
 
+// A graph's descriptor with every field made wave-uniform (scalar
+// registers): loop bounds such as n then compile to scalar compares instead
+// of per-lane compares with exec-mask bookkeeping.
+__device__ __forceinline__ GDesc load_desc(const GDesc *p)
+{
+    GDesc d = *p;
+    d.n = uni(d.n);
+    d.m = uni(d.m);
+    d.off_src = uni(d.off_src);
+    d.off_dst = uni(d.off_dst);
+    d.off_deg = uni(d.off_deg);
+    d.off_cover = uni(d.off_cover);
+    d.instance = uni(d.instance);
+    d.k = uni(d.k);
+    d.U = uni(d.U);
+    d.seq = uni(d.seq);
+    d.max_nodes = ((uint64_t)uni((uint32_t)(d.max_nodes >> 32)) << 32) | uni((uint32_t)d.max_nodes);
+    return d;
+}
+
 // ---------------------------------------------------------------------------
 // the persistent solve kernel
 // ---------------------------------------------------------------------------
@@ -1222,7 +1243,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
                 } else {
                     g = a.version == 1 ? q.root : (q.root >> a.start_depth);
                 }
-                gd = a.gdesc[g];
+                gd = load_desc(a.gdesc + g);
                 w.n = gd.n;
                 w.src = a.gbuf + gd.off_src;
                 w.dst = a.gbuf + gd.off_dst;
