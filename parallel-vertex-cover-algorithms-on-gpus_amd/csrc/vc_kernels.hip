@@ -416,12 +416,15 @@ __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
             const int x = x0 + lane;
             uint32_t leaf = kNone;
             if (x < total) {
-                // owner: last entry j < cnt with xoff[j] <= x
+                // owner: last entry j < cnt with xoff[j] <= x (a leaf pass
+                // usually applies one vertex: no search then)
                 int pos = 0;
-#pragma unroll 1
-                for (int step = 32; step >= 1; step >>= 1) {
-                    const int cand = pos + step;
-                    if (cand < (int)cnt && xoff[cand] <= x) pos = cand;
+                if (cnt > 1u) {
+#pragma unroll
+                    for (int step = 32; step >= 1; step >>= 1) {
+                        const int cand = pos + step;
+                        if (cand < (int)cnt && xoff[cand] <= x) pos = cand;
+                    }
                 }
                 const uint32_t t = w.dst[xbase[pos] + (x - xoff[pos])];
                 if (w.deg(t) != -1 && w.drop(t, (uint32_t)xvert[pos]) == 2) leaf = t;
