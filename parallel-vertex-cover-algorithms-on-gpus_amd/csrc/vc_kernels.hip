@@ -580,13 +580,19 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
 }
 
 // High-degree rule (src/helperFunctions.cuh:299-358): v goes into the cover
-// when deg(v) + d + (deleted so far in this call) >= T.
+// when deg(v) + d + (deleted so far in this call) >= T.  A pass also keeps
+// the largest degree it did not mark; degrees only fall, so when that value
+// cannot reach T with the new count, the next pass would mark nothing and is
+// not run.  st then describes the degrees before the last apply: the caller
+// runs the triangle scan again after any high-degree deletion (nh > 0), which
+// refreshes it.
 template <class N>
 __device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st, uint32_t &nc)
 {
     uint32_t total = 0;
     for (;;) {
         uint32_t L = 0;
+        int rest = 0;  // largest degree left unmarked by this pass (per lane)
         lanes_reset(st);
         for (uint32_t base = 0; base < w.n; base += kWave) {
             uint32_t v = base + lane_id();
@@ -594,13 +600,17 @@ __device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st, uin
             if (v < w.n) {
                 int dv = w.deg(v);
                 lanes_add(st, dv, v);
-                if (dv > 0 && (uint32_t)dv + d + total >= T) cand = v;
+                if (dv > 0 && (uint32_t)dv + d + total >= T)
+                    cand = v;
+                else
+                    rest = max(rest, dv);
             }
             L = append(w, L, cand, false);
         }
         if (L == 0) break;
         apply_list(w, L, nc);
         total += L;
+        if ((uint32_t)wave_max(rest) + d + total < T) break;
     }
     return total;
 }
