@@ -134,7 +134,10 @@ vc_status vc_ctx_destroy(vc_ctx *ctx);
 /* Upload `count` graphs into one device-resident graph set (one HBM buffer).
  * count == 1 replaces allocateGraph (src/CSRGraphRep.cuh:3-27); larger
  * counts are the batched form.  Graphs are checked (check_graph,
- * src/auxFunctions.cpp:308-354) and rejected with VC_EBADGRAPH. */
+ * src/auxFunctions.cpp:308-354, plus what the kernels rely on: offsets
+ * consistent with degree[] and edgeNum, each adjacency sorted ascending as
+ * createCSRGraphFromFile leaves it, and every edge listed from both ends)
+ * and rejected with VC_EBADGRAPH. */
 vc_status vc_graphs_upload(vc_ctx *ctx, int count, const vc_csr *graphs, vc_graphs **out);
 vc_status vc_graphs_free(vc_ctx *ctx, vc_graphs *set);
 
@@ -221,7 +224,9 @@ vc_status vc_load_graph_file(const char *path, vc_csr_owned *out);
 vc_status vc_csr_from_edges(uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *ev, vc_csr_owned *out);
 void vc_csr_free(vc_csr_owned *g);
 
-/* check_graph (src/auxFunctions.cpp:308-354): 1 = valid. */
+/* check_graph (src/auxFunctions.cpp:308-354): 1 = valid (no duplicate
+ * neighbour in any order, no self-loop, indices and offsets in range),
+ * computed in O(n + m) instead of the reference's n x n matrix. */
 int vc_check_graph(const vc_csr *g);
 
 /* RemoveMaxApproximateMVC (src/auxFunctions.cpp:211-256).  Returns U; if

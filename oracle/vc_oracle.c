@@ -113,13 +113,38 @@ void or_graph_free(or_graph *g)
  * neighbours in a row -- same verdict in O(m). */
 int or_check_graph(const or_graph *g)
 {
-    for (uint32_t v = 0; v < g->n; ++v) {
+    /* check_graph (src/auxFunctions.cpp:308-354): an n x n adjacency matrix
+     * marks every listed (vertex, neighbour) pair; a pair marked twice (a
+     * duplicate, in any list order) fails, then any mark on the diagonal (a
+     * self-loop) fails.  Bits instead of the reference's unsigned ints; a
+     * neighbour index >= n (which the reference would write past its
+     * matrix with) also fails. */
+    const uint64_t n = g->n;
+    if (n == 0) return 1;
+    uint8_t *mat = (uint8_t *)calloc((size_t)((n * n + 7) / 8), 1);
+    if (!mat) return 0;
+    int ok = 1;
+    for (uint32_t v = 0; v < g->n && ok; ++v) {
         for (uint32_t e = g->srcPtr[v]; e < g->srcPtr[v + 1]; ++e) {
-            if (g->dst[e] == v) return 0;
-            if (e > g->srcPtr[v] && g->dst[e] == g->dst[e - 1]) return 0;
+            const uint32_t u = g->dst[e];
+            if (u >= g->n) {
+                ok = 0;
+                break;
+            }
+            const uint64_t bit = (uint64_t)v * n + u;
+            if (mat[bit >> 3] & (1u << (bit & 7))) {
+                ok = 0;
+                break;
+            }
+            mat[bit >> 3] |= (uint8_t)(1u << (bit & 7));
         }
     }
-    return 1;
+    for (uint32_t v = 0; v < g->n && ok; ++v) {
+        const uint64_t bit = (uint64_t)v * n + v;
+        if (mat[bit >> 3] & (1u << (bit & 7))) ok = 0;
+    }
+    free(mat);
+    return ok;
 }
 
 /* ------------------------------------------------------------------ */

@@ -17,8 +17,11 @@
  *     brute force on tiny graphs, closed forms, an independent MILP solver.
  *   or_remove_max_approx / or_remove_edge_approx : oracle/_ref (bit-exact U;
  *     RemoveEdge with the reference's time(0) seed replaced by a fixed seed).
- *   or_graph_load / or_check_graph : oracle/_ref (same CSR arrays), and the
- *     reference's data files.
+ *   or_graph_load : oracle/_ref (same CSR arrays) and the reference's data
+ *     files.
+ *   or_check_graph : the reference's data files, and the definition (no list
+ *     repeats a vertex or names its own vertex) checked by brute force on
+ *     random unsorted lists, which the reference's loader never produces.
  */
 #ifndef VC_ORACLE_H
 #define VC_ORACLE_H

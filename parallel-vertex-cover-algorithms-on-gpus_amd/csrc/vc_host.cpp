@@ -180,28 +180,45 @@ vc_status vc_csr_from_edges(uint32_t n, uint32_t m, const uint32_t *eu, const ui
     return build_csr(n, m, eu, ev, out);
 }
 
+// createCSRGraphFromFile (src/auxFunctions.cpp:129-209): header "n m", then
+// m lines "u v", 1-indexed.  The file is read whole and its fields parsed
+// directly (whitespace-separated unsigned decimals, as "%u" reads a
+// well-formed file); a short or malformed file is VC_EBADGRAPH instead of the
+// reference's reads of stale values.
 vc_status vc_load_graph_file(const char *path, vc_csr_owned *out)
 {
     if (!path || !out) return VC_EINVAL;
     std::memset(out, 0, sizeof(*out));
-    FILE *fp = std::fopen(path, "r");
+    FILE *fp = std::fopen(path, "rb");
     if (!fp) return VC_EINVAL;
-    unsigned n = 0, m = 0;
-    if (std::fscanf(fp, "%u%u", &n, &m) != 2) {
-        std::fclose(fp);
-        return VC_EBADGRAPH;
-    }
-    std::vector<uint32_t> eu(m), ev(m);
-    for (unsigned i = 0; i < m; ++i) {
-        unsigned a, b;
-        if (std::fscanf(fp, "%u%u", &a, &b) != 2 || a == 0 || b == 0) {
-            std::fclose(fp);
-            return VC_EBADGRAPH;
+    std::vector<char> buf;
+    char chunk[1 << 16];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof(chunk), fp)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+    std::fclose(fp);
+    const char *p = buf.data(), *end = buf.data() + buf.size();
+    auto next = [&](uint32_t &x) -> bool {
+        while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r' || *p == '\v' || *p == '\f')) ++p;
+        if (p < end && *p == '+') ++p;
+        if (p >= end || *p < '0' || *p > '9') return false;
+        uint64_t v = 0;
+        while (p < end && *p >= '0' && *p <= '9') {
+            v = v * 10u + (uint64_t)(*p - '0');
+            if (v > 0xFFFFFFFFull) return false;
+            ++p;
         }
+        x = (uint32_t)v;
+        return true;
+    };
+    uint32_t n = 0, m = 0;
+    if (!next(n) || !next(m)) return VC_EBADGRAPH;
+    std::vector<uint32_t> eu(m), ev(m);
+    for (uint32_t i = 0; i < m; ++i) {
+        uint32_t a, b;
+        if (!next(a) || !next(b) || a == 0 || b == 0) return VC_EBADGRAPH;
         eu[i] = a - 1u;  // 1-indexed file (src/auxFunctions.cpp:151-153)
         ev[i] = b - 1u;
     }
-    std::fclose(fp);
     return build_csr(n, m, eu.data(), ev.data(), out);
 }
 
@@ -212,13 +229,26 @@ void vc_csr_free(vc_csr_owned *g)
     std::memset(g, 0, sizeof(*g));
 }
 
+// check_graph (src/auxFunctions.cpp:308-354): no vertex lists a neighbour
+// twice (in any order) and none lists itself.  The reference marks an n x n
+// matrix (O(n^2) memory and time); a last-seen marker per vertex gives the
+// same answer in O(n + m).  Offsets that leave the edge array and indices
+// out of range (which the reference would read past) also fail.
 int vc_check_graph(const vc_csr *g)
 {
     if (!g) return 0;
-    for (uint32_t v = 0; v < g->vertexNum; ++v) {
-        for (uint32_t e = g->srcPtr[v]; e < g->srcPtr[v + 1]; ++e) {
-            if (g->dst[e] == v || g->dst[e] >= g->vertexNum) return 0;
-            if (e > g->srcPtr[v] && g->dst[e] == g->dst[e - 1]) return 0;
+    const uint32_t n = g->vertexNum;
+    if (n == 0) return 1;
+    if (!g->srcPtr || (g->srcPtr[n] > 0 && !g->dst)) return 0;
+    const uint64_t slots = 2ull * g->edgeNum;
+    std::vector<uint32_t> seen(n, 0xFFFFFFFFu);
+    for (uint32_t v = 0; v < n; ++v) {
+        const uint32_t b = g->srcPtr[v], e = g->srcPtr[v + 1];
+        if (e < b || e > slots) return 0;
+        for (uint32_t x = b; x < e; ++x) {
+            const uint32_t u = g->dst[x];
+            if (u >= n || u == v || seen[u] == v) return 0;
+            seen[u] = v;
         }
     }
     return 1;

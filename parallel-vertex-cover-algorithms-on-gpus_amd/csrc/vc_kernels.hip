@@ -1578,6 +1578,13 @@ bool csr_is_valid(const vc_csr &g)
             if (x > b && g.dst[x - 1] >= u) return false;  // unsorted or duplicate
         }
     }
+    // undirected: every listed edge is listed from its other end too (the
+    // neighbour sketch and the degree updates rely on it)
+    for (uint32_t v = 0; v < n; ++v)
+        for (uint32_t x = g.srcPtr[v]; x < g.srcPtr[v + 1]; ++x) {
+            const uint32_t u = g.dst[x];
+            if (!std::binary_search(g.dst + g.srcPtr[u], g.dst + g.srcPtr[u + 1], v)) return false;
+        }
     return true;
 }
 

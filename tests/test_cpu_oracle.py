@@ -332,6 +332,60 @@ def test_product_rejects_bad_graphs(tmp_path):
         vcgpu.load_graph_file(str(p3))
 
 
+def test_product_check_graph_matches_reference_semantics():
+    """vc_check_graph (a last-seen marker, O(n + m)) answers exactly as the
+    reference's n x n matrix check, restated in the oracle, on adjacency
+    lists with duplicates in any order and self-loops at random places."""
+    vcgpu = _vc()
+    rng = np.random.Generator(np.random.PCG64(5))
+    outcomes = set()
+    for _ in range(400):
+        n = int(rng.integers(1, 25))
+        lists = []
+        for v in range(n):
+            k = int(rng.integers(0, min(n, 7)))
+            if rng.random() < 0.5:
+                others = [u for u in range(n) if u != v]
+                nb = list(rng.choice(others, size=min(k, len(others)), replace=False)) if others else []
+            else:
+                nb = list(rng.integers(0, n, size=k))
+            rng.shuffle(nb)
+            lists.append([int(u) for u in nb])
+        deg = np.array([len(x) for x in lists], np.int32)
+        src = np.zeros(n + 1, np.uint32)
+        src[1:] = np.cumsum(deg)
+        total = int(src[-1])
+        m = (total + 1) // 2
+        dst = np.zeros(max(2 * m, 1), np.uint32)
+        dst[:total] = [u for x in lists for u in x]
+        # the definition: no list repeats a vertex or names its own vertex
+        brute = all(len(set(x)) == len(x) and v not in x for v, x in enumerate(lists))
+        want = O.check_graph(O.Graph(n, m, src, dst, deg))
+        got = vcgpu.check_graph(vcgpu.CSR(n, m, src, dst, deg))
+        assert want == brute and got == brute, lists
+        outcomes.add(brute)
+    assert outcomes == {True, False}
+
+
+def test_product_loader_reads_what_the_reference_reads(tmp_path):
+    """Tabs, CR LF line ends, blank and trailing lines and a header with a
+    trailing space (the .mtx files) parse as the reference's fscanf loop
+    parses them; a file with fewer edges than its header says is an error."""
+    vcgpu = _vc()
+    n, e = 60, graphgen.erdos_renyi(60, 0.1, 11)
+    lines = [f"{n} {len(e)} "] + [f"{u + 1}\t{v + 1}" for u, v in e]
+    p = tmp_path / "odd.txt"
+    p.write_bytes(("\r\n".join(lines) + "\r\n\n\n").encode())
+    a = vcgpu.load_graph_file(str(p))
+    b = O.load(str(p))
+    assert (a.n, a.m) == (b.n, b.m) == (n, len(e))
+    assert np.array_equal(a.srcPtr, b.srcPtr) and np.array_equal(a.dst, b.dst)
+    short = tmp_path / "short.txt"
+    short.write_text("\n".join([f"{n} {len(e) + 3}"] + [f"{u + 1} {v + 1}" for u, v in e]) + "\n")
+    with pytest.raises(vcgpu.VCError):
+        vcgpu.load_graph_file(str(short))
+
+
 @pytest.mark.parametrize("vec", VECS[:40], ids=lambda v: v["key"])
 def test_product_upper_bounds_match_reference(vec):
     """vc_approx_remove_max / vc_approx_remove_edge give the reference's U

@@ -259,3 +259,22 @@ def test_autotuner_picks_a_working_shape(ctx):
     res, info, st = ctx.solve(gs, jobs, best)
     for (name, n, e), r in zip(cases, res):
         assert r.solved and r.minimum == O.mvc(O.from_edges(n, e))[0], name
+
+
+def test_upload_rejects_what_the_kernels_cannot_solve(ctx):
+    """The kernels rely on sorted, symmetric adjacency (binary search,
+    neighbour sums): an edge listed from one end only, or an unsorted list,
+    is rejected at upload with VC_EBADGRAPH."""
+    g = vcgpu.csr_from_edges(4, [(0, 1), (1, 2), (2, 3)])
+    one_sided = vcgpu.CSR(g.n, g.m, g.srcPtr.copy(), g.dst.copy(), g.degree.copy())
+    one_sided.dst[one_sided.srcPtr[0]] = 3  # 0 lists 3, 3 does not list 0
+    unsorted = vcgpu.CSR(g.n, g.m, g.srcPtr.copy(), g.dst.copy(), g.degree.copy())
+    b = int(unsorted.srcPtr[1])
+    unsorted.dst[b], unsorted.dst[b + 1] = unsorted.dst[b + 1], unsorted.dst[b]
+    for bad in (one_sided, unsorted):
+        with pytest.raises(vcgpu.VCError) as ei:
+            ctx.upload(bad)
+        assert ei.value.status == vcgpu.VC_EBADGRAPH
+    gs = ctx.upload(g)  # the well-formed graph still uploads and solves
+    (r,), _, _ = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, 2, 0), P())
+    assert r.solved and r.minimum == 2
