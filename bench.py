@@ -383,7 +383,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(independent graphs would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              f"(independent graphs would shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     ctx = vcgpu.Context(0)
@@ -488,8 +488,11 @@ def main():
         U, _ = vcgpu.upper_bound(g, seed=0)
         gset = ctx.upload(g)
         if args.workload == "single_ba":
-            import oracle as O  # only to know opt for k = opt-1 / opt; checked, not measured
-            opt = O.mvc(O.from_edges(n, e))[0]
+            # opt for k = opt - 1 / opt from an exact MVC solve (not timed)
+            (r0,), _, _ = ctx.solve(gset, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), params, want_covers=False)
+            if not r0.solved:
+                raise RuntimeError("single_ba: the MVC solve that fixes k did not finish")
+            opt = r0.minimum
             jobs_list = [[vcgpu.Job(vcgpu.VC_PVC, opt - 1, U, 0)], [vcgpu.Job(vcgpu.VC_PVC, opt, U, 0)]]
         else:
             jobs_list = [[vcgpu.Job(vcgpu.VC_MVC, 0, U, 0)]]
