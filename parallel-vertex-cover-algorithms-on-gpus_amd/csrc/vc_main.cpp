@@ -47,7 +47,7 @@ void usage()
                  "   -i <Instance>                 mvc or pvc (default mvc)\n"
                  "   -k <parameter>                k for pvc (default 0)\n"
                  "   -q <Global Work List Size>    worklist slots, power of 2 (default: sized to the grid)\n"
-                 "   -t <Global Work List Threshold> donate while queued < t * size, 0.0-1.0 (default: half the workers)\n"
+                 "   -t <Global Work List Threshold> donate while queued < t * size, 0.0-1.0 (default: only while workers wait)\n"
                  "   -d <Starting Depth>           static split depth for stackonly (default 4)\n"
                  "   -b <Block Dimension>          threads per workgroup, multiple of 64, <= 256 (default 256)\n"
                  "   -g <Use Global Memory>        0 = node storage in LDS, 1 = in HBM (default: LDS when it fits)\n"

@@ -1,0 +1,76 @@
+"""The drop-in command line (`vc`, csrc/vc_main.cpp): the reference's flags
+(src/config.h:178-243), its stdout lines and its Results/Results.csv row
+(src/auxFunctions.cpp:580-614)."""
+import csv
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import graphgen
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+VC = os.path.join(os.path.dirname(HERE), "parallel-vertex-cover-algorithms-on-gpus_amd", "vc")
+
+
+def run_vc(args, cwd, seed=7):
+    env = dict(os.environ, VC_SEED=str(seed))
+    return subprocess.run([VC] + args, cwd=str(cwd), capture_output=True, text=True, timeout=120, env=env)
+
+
+def _graph(tmp_path, n=80, p=0.1, seed=3, name="g.txt"):
+    e = graphgen.erdos_renyi(n, p, seed)
+    path = tmp_path / name
+    graphgen.write_graph_file(str(path), n, e)
+    return path, n, e
+
+
+def test_cli_upper_bounds_are_the_references(tmp_path):
+    """The two approximation lines carry the reference's U values (the oracle
+    restates RemoveMax / RemoveEdge bit for bit); -v sequential, the
+    reference's CPU solver, is refused after them."""
+    path, n, e = _graph(tmp_path)
+    r = run_vc(["-f", path.name, "-v", "sequential"], tmp_path, seed=7)
+    assert r.returncode == 2 and "not part of this GPU build" in r.stderr
+    g = O.from_edges(n, e)
+    got_max = int(re.search(r"Approximate Remove Max Minimum is: (\d+)", r.stdout).group(1))
+    got_edge = int(re.search(r"Approximate Remove Edge Minimum is: (\d+)", r.stdout).group(1))
+    assert got_max == O.remove_max(g)
+    assert got_edge == O.remove_edge(g, 7)
+    assert "Graph file: g.txt" in r.stdout
+
+
+def test_cli_rejects_bad_input(tmp_path):
+    path, _, _ = _graph(tmp_path)
+    assert "Unrecognized option" in run_vc(["-x"], tmp_path).stderr
+    assert run_vc(["-f", path.name, "-v", "fastest"], tmp_path).returncode == 2
+    r = run_vc(["-f", path.name, "-q", "1000"], tmp_path)
+    assert r.returncode == 2 and "power of 2" in r.stderr
+    dup = tmp_path / "dup.txt"
+    dup.write_text("3 2\n1 2\n2 1\n")
+    r = run_vc(["-f", dup.name], tmp_path)
+    assert r.returncode == 2 and "graph check failed" in r.stderr
+    assert run_vc(["-f", "missing.txt"], tmp_path).returncode == 2
+
+
+@pytest.mark.gpu
+def test_cli_solves_and_appends_the_results_row(tmp_path):
+    """MVC and PVC through the CLI: the reference's answer lines, and one
+    22-column Results.csv row per run (src/auxFunctions.cpp:589-593)."""
+    path, n, e = _graph(tmp_path, n=70, p=0.08, seed=21)
+    opt, _, _ = O.mvc(O.from_edges(n, e))
+    r = run_vc(["-f", path.name, "-o", "run1"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert int(re.search(r"Size of minimum vertex cover: (\d+)", r.stdout).group(1)) == opt
+    for k, word in ((opt - 1, "greater than"), (opt, "less than or equal to")):
+        r = run_vc(["-f", path.name, "-i", "pvc", "-k", str(k), "-v", "stackonly", "-o", f"pvc{k}"], tmp_path)
+        assert r.returncode == 0, r.stderr
+        assert f"Minimum is {word} K: {k}" in r.stdout
+    rows = list(csv.reader(open(tmp_path / "Results" / "Results.csv")))
+    assert len(rows) == 3 and all(len(row) == 22 for row in rows)
+    assert rows[0][0] == "run1" and rows[0][4] == "mvc" and int(rows[0][18]) == opt
+    assert rows[1][4] == "pvc" and rows[1][5] == "stackonly" and int(rows[1][20]) == 0
+    assert int(rows[2][20]) == 1
