@@ -496,7 +496,7 @@ __device__ uint32_t rule_leaf(const N &w, uint32_t &nc, bool &full)
     for (;;) {
         uint32_t L = 0;
         if (full || nc > w.ncap) {
-            for (uint32_t base = 0; base < w.n; base += kWave) {
+            for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
                 const uint32_t v = base + lane_id();
                 const int dv = v < w.n ? w.deg(v) : -1;
                 L = append(w, L, leaf_mark(w, v, dv), true);
@@ -548,7 +548,7 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
     for (;;) {
         uint32_t L = 0;
         lanes_reset(st);
-        for (uint32_t base = 0; base < w.n; base += kWave) {
+        for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
             uint32_t v = base + lane_id();
             uint32_t c1 = kNone, c2 = kNone;
             int dv = v < w.n ? w.deg(v) : -1;
@@ -561,7 +561,7 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
                 // skipped when a neighbour has degree 1, or degree 2 and a
                 // lower index (src/helperFunctions.cuh:379-385)
                 const int d1 = w.deg(n1), d2 = w.deg(n2);
-                const bool skip = d1 == 1 || (d1 == 2 && n1 < v) || d2 == 1 || (d2 == 2 && n2 < v);
+                const bool skip = (d1 == 1) | ((d1 == 2) & (n1 < v)) | (d2 == 1) | ((d2 == 2) & (n2 < v));
                 if (!skip && adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
                     c1 = n1;
                     c2 = n2;
@@ -594,7 +594,7 @@ __device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st, uin
         uint32_t L = 0;
         int rest = 0;  // largest degree left unmarked by this pass (per lane)
         lanes_reset(st);
-        for (uint32_t base = 0; base < w.n; base += kWave) {
+        for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
             uint32_t v = base + lane_id();
             uint32_t cand = kNone;
             if (v < w.n) {
@@ -873,7 +873,7 @@ template <class N>
 __device__ void write_cover(const SolveArgs &a, const N &w, const GDesc &gd)
 {
     uint32_t *cw = a.covers + gd.off_cover;
-    for (uint32_t base = 0; base < w.n; base += kWave) {
+    for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
         uint32_t v = base + lane_id();
         uint64_t m = __ballot(v < w.n && w.deg(v) == -1);
         if (lane_id() == 0) {
