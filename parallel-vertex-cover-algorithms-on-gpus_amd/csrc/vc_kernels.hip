@@ -321,6 +321,16 @@ __device__ __forceinline__ uint32_t append(const N &w, uint32_t L, uint32_t cand
     return uni(min(L + (uint32_t)__popcll(m), w.lcap));
 }
 
+// Degree of v, or -1 for v >= n (padding lanes of a scan chunk, empty
+// candidate lanes).  The read goes to a clamped index, so a chunk needs no
+// branch around it; n >= 1 wherever a chunk runs.
+template <class N>
+__device__ __forceinline__ int deg_or_none(const N &w, uint32_t v, uint32_t n)
+{
+    const int d = w.deg(v < n ? v : 0u);
+    return v < n ? d : -1;
+}
+
 // Leaf candidates: every vertex whose degree falls from 2 to 1 is appended
 // (lanes with t == kNone add nothing).  Degrees never grow along a path, so
 // a vertex enters at most once between two leaf passes; a count beyond ncap
@@ -479,14 +489,12 @@ __device__ __forceinline__ void lanes_add(Lanes &st, int dv, uint32_t v)
 template <class N>
 __device__ __forceinline__ uint32_t leaf_mark(const N &w, uint32_t v, int dv)
 {
-    if (dv == 1) {
-        const uint32_t u = w.s1(v);
-        if (u < w.n) {
-            const int du = w.deg(u);
-            if (du > 1 || (du == 1 && u < v)) return u;
-        }
-    }
-    return kNone;
+    // branch-free: every lane reads at a valid index, the result is selected
+    const bool leaf = dv == 1;  // false for padding lanes (dv == -1)
+    const uint32_t u = w.s1(leaf ? v : 0u);
+    const int du = deg_or_none(w, u, w.n);
+    const bool mark = leaf & ((du > 1) | ((du == 1) & (u < v)));
+    return mark ? u : kNone;
 }
 
 template <class N>
@@ -498,14 +506,14 @@ __device__ uint32_t rule_leaf(const N &w, uint32_t &nc, bool &full)
         if (full || nc > w.ncap) {
             for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
                 const uint32_t v = base + lane_id();
-                const int dv = v < w.n ? w.deg(v) : -1;
+                const int dv = deg_or_none(w, v, nv);
                 L = append(w, L, leaf_mark(w, v, dv), true);
             }
         } else {
             for (uint32_t i0 = 0; i0 < nc; i0 += kWave) {
                 const uint32_t i = i0 + lane_id();
                 const uint32_t v = i < nc ? (uint32_t)w.cand[i] : kNone;
-                const int dv = v != kNone ? w.deg(v) : -1;
+                const int dv = deg_or_none(w, v, w.n);
                 L = append(w, L, leaf_mark(w, v, dv), true);
             }
         }
@@ -551,7 +559,7 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
         for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
             uint32_t v = base + lane_id();
             uint32_t c1 = kNone, c2 = kNone;
-            int dv = v < w.n ? w.deg(v) : -1;
+            int dv = deg_or_none(w, v, nv);
             lanes_add(st, dv, v);
             using S2 = typename N::s2_t;
             uint32_t n1, n2;
@@ -595,17 +603,12 @@ __device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st, uin
         int rest = 0;  // largest degree left unmarked by this pass (per lane)
         lanes_reset(st);
         for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
-            uint32_t v = base + lane_id();
-            uint32_t cand = kNone;
-            if (v < w.n) {
-                int dv = w.deg(v);
-                lanes_add(st, dv, v);
-                if (dv > 0 && (uint32_t)dv + d + total >= T)
-                    cand = v;
-                else
-                    rest = max(rest, dv);
-            }
-            L = append(w, L, cand, false);
+            const uint32_t v = base + lane_id();
+            const int dv = deg_or_none(w, v, nv);  // -1 past n: adds nothing below
+            lanes_add(st, dv, v);
+            const bool hit = (dv > 0) & ((uint32_t)dv + d + total >= T);
+            rest = hit ? rest : max(rest, dv);
+            L = append(w, L, hit ? v : kNone, false);
         }
         if (L == 0) break;
         apply_list(w, L, nc);
