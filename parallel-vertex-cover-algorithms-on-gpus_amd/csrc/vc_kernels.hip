@@ -340,6 +340,7 @@ template <class N>
 __device__ __forceinline__ uint32_t add_leaf_cands(const N &w, uint32_t nc, uint32_t t)
 {
     const uint64_t m = __ballot(t != kNone);
+    if (m == 0ull) return nc;  // the common case: a wave-uniform exit
     if (t != kNone) {
         const uint32_t pos = nc + lanes_below(m);
         if (pos < w.ncap) w.cand[pos] = (typename N::cand_t)t;
