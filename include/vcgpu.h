@@ -220,6 +220,13 @@ typedef struct {
 /* createCSRGraphFromFile (src/auxFunctions.cpp:129-209): header "n m" then
  * m 1-indexed edges; adjacency sorted.  Errors are returned, not asserted. */
 vc_status vc_load_graph_file(const char *path, vc_csr_owned *out);
+/* The same loader with the first vertex numbered index_base (0 or 1; other
+ * values are VC_EINVAL).  index_base 1 is vc_load_graph_file.  index_base 0
+ * reads the reference's 0-indexed files (test/test_graphs/formated_* and
+ * lastfm-asia.txt, whose MVC test/runGraphs.py:21-29 lists, and
+ * data/GraphDatabase/Extra/{60,120}-cell.txt), which its own loader
+ * (src/auxFunctions.cpp:151-153) shifts to vertex -1. */
+vc_status vc_load_graph_file_indexed(const char *path, uint32_t index_base, vc_csr_owned *out);
 /* Same CSR from a 0-indexed edge list. */
 vc_status vc_csr_from_edges(uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *ev, vc_csr_owned *out);
 void vc_csr_free(vc_csr_owned *g);

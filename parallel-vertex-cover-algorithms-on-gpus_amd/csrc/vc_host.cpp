@@ -187,7 +187,15 @@ vc_status vc_csr_from_edges(uint32_t n, uint32_t m, const uint32_t *eu, const ui
 // reference's reads of stale values.
 vc_status vc_load_graph_file(const char *path, vc_csr_owned *out)
 {
-    if (!path || !out) return VC_EINVAL;
+    return vc_load_graph_file_indexed(path, 1u, out);
+}
+
+// index_base 1 is the reference's format; index_base 0 reads the files the
+// reference ships 0-indexed (test/test_graphs/formated_*, lastfm-asia.txt,
+// data/GraphDatabase/Extra/*-cell.txt), which its own loader shifts to -1.
+vc_status vc_load_graph_file_indexed(const char *path, uint32_t index_base, vc_csr_owned *out)
+{
+    if (!path || !out || index_base > 1u) return VC_EINVAL;
     std::memset(out, 0, sizeof(*out));
     FILE *fp = std::fopen(path, "rb");
     if (!fp) return VC_EINVAL;
@@ -215,9 +223,9 @@ vc_status vc_load_graph_file(const char *path, vc_csr_owned *out)
     std::vector<uint32_t> eu(m), ev(m);
     for (uint32_t i = 0; i < m; ++i) {
         uint32_t a, b;
-        if (!next(a) || !next(b) || a == 0 || b == 0) return VC_EBADGRAPH;
-        eu[i] = a - 1u;  // 1-indexed file (src/auxFunctions.cpp:151-153)
-        ev[i] = b - 1u;
+        if (!next(a) || !next(b) || a < index_base || b < index_base) return VC_EBADGRAPH;
+        eu[i] = a - index_base;  // shift of src/auxFunctions.cpp:151-153
+        ev[i] = b - index_base;
     }
     return build_csr(n, m, eu.data(), ev.data(), out);
 }

@@ -35,6 +35,7 @@ struct Config {
     const char *graphFileName = "GraphInput.txt";
     bool userListSize = false;
     bool userThreshold = false;
+    unsigned indexBase = 1;
 };
 
 void usage()
@@ -51,7 +52,8 @@ void usage()
                  "   -d <Starting Depth>           static split depth for stackonly (default 4)\n"
                  "   -b <Block Dimension>          threads per workgroup, multiple of 64, <= 256 (default 256)\n"
                  "   -g <Use Global Memory>        0 = node storage in LDS, 1 = in HBM (default: LDS when it fits)\n"
-                 "   -n <Number of Blocks>         workgroups (default occupancy x CUs)\n\n"
+                 "   -n <Number of Blocks>         workgroups (default occupancy x CUs)\n"
+                 "   -z                            the graph file's vertices start at 0 (formated_*, lastfm-asia, *-cell)\n\n"
                  "   VC_SEED=<u32> fixes the RemoveEdge seed (default time(0), as the reference).\n\n");
 }
 
@@ -63,7 +65,7 @@ const char *as_string(Instance i) { return i == MVC ? "mvc" : "pvc"; }
 bool parse_args(int argc, char **argv, Config &c)
 {
     int opt;
-    while ((opt = getopt(argc, argv, "v:f:o:i:k:q:t:d:b:g:n:h")) >= 0) {
+    while ((opt = getopt(argc, argv, "v:f:o:i:k:q:t:d:b:g:n:zh")) >= 0) {
         switch (opt) {
         case 'v':
             if (!std::strcmp(optarg, "stackonly")) c.version = STACK_ONLY;
@@ -91,6 +93,7 @@ bool parse_args(int argc, char **argv, Config &c)
         case 'b': c.blockDim = (unsigned)std::atoi(optarg); break;
         case 'g': c.userDefMemory = true; c.useGlobalMemory = std::atoi(optarg) != 0; break;
         case 'n': c.numBlocks = (unsigned)std::atoi(optarg); break;
+        case 'z': c.indexBase = 0; break;
         case 'h': usage(); std::exit(0);
         default: std::fprintf(stderr, "\nUnrecognized option!\n"); return false;
         }
@@ -139,7 +142,7 @@ int main(int argc, char **argv)
     std::printf("\nUUID: %s\n", c.outputFilePrefix);
 
     vc_csr_owned owned;
-    vc_status st = vc_load_graph_file(c.graphFileName, &owned);
+    vc_status st = vc_load_graph_file_indexed(c.graphFileName, c.indexBase, &owned);
     if (st != VC_OK) {
         std::fprintf(stderr, "cannot load %s: %s\n", c.graphFileName, vc_status_string(st));
         return 2;

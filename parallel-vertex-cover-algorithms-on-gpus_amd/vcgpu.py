@@ -73,8 +73,8 @@ class _ServerConfig(C.Structure):
 # Every symbol include/vcgpu.h declares.
 EXPORTS = [
     "vc_ctx_create", "vc_ctx_destroy", "vc_graphs_upload", "vc_graphs_free", "vc_solve", "vc_occupancy",
-    "vc_load_graph_file", "vc_csr_from_edges", "vc_csr_free", "vc_check_graph", "vc_approx_remove_max",
-    "vc_approx_remove_edge", "vc_status_string", "vc_server_start", "vc_server_submit", "vc_server_poll",
+    "vc_load_graph_file", "vc_load_graph_file_indexed", "vc_csr_from_edges", "vc_csr_free", "vc_check_graph",
+    "vc_approx_remove_max", "vc_approx_remove_edge", "vc_status_string", "vc_server_start", "vc_server_submit", "vc_server_poll",
     "vc_server_wait", "vc_server_stop", "vc_now_ns", "vc_autotune",
 ]
 VC_EAGAIN = 8
@@ -107,6 +107,7 @@ def lib():
         L.vc_autotune.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_Job), C.POINTER(_Params), C.POINTER(_Params),
                                   C.POINTER(C.c_float)]
         L.vc_load_graph_file.argtypes = [C.c_char_p, C.POINTER(_CsrOwned)]
+        L.vc_load_graph_file_indexed.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(_CsrOwned)]
         L.vc_csr_from_edges.argtypes = [C.c_uint32, C.c_uint32, _u32p, _u32p, C.POINTER(_CsrOwned)]
         L.vc_csr_free.argtypes = [C.POINTER(_CsrOwned)]
         L.vc_check_graph.argtypes = [C.POINTER(_Csr)]
@@ -166,10 +167,15 @@ def _own(o: _CsrOwned) -> CSR:
     return CSR(n, m, src.astype(np.uint32), dst.astype(np.uint32), deg.astype(np.int32))
 
 
-def load_graph_file(path: str) -> CSR:
-    """createCSRGraphFromFile (src/auxFunctions.cpp:129-209)."""
+def load_graph_file(path: str, index_base: int = 1) -> CSR:
+    """createCSRGraphFromFile (src/auxFunctions.cpp:129-209).  index_base 1 is
+    the reference's format; 0 reads its 0-indexed files (formated_*,
+    lastfm-asia, *-cell)."""
     o = _CsrOwned()
-    _check(lib().vc_load_graph_file(path.encode(), C.byref(o)), f"load {path}")
+    if index_base == 1:
+        _check(lib().vc_load_graph_file(path.encode(), C.byref(o)), f"load {path}")
+    else:
+        _check(lib().vc_load_graph_file_indexed(path.encode(), index_base, C.byref(o)), f"load {path}")
     return _own(o)
 
 

@@ -13,7 +13,8 @@
                            stops if the two disagree.
 
 Run from the repository root:  python tests/golden/make_golden.py [--with-slow]
-(--with-slow adds the 200-vertex Erdos-Renyi workload graph, ~5 min of CPU).
+(--with-slow adds the 200-vertex Erdos-Renyi workload graph, ~5 min of CPU;
+--only-known rewrites graphs/ and known_answers.json only).
 """
 import argparse
 import io
@@ -34,7 +35,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_GRAPHS = "/root/reference/test/test_graphs"
 
 # Known answers of test/runGraphs.py:8-30 for graphs shipped in the reference
-# in its 1-indexed format and small enough to keep as fixtures.
+# and small enough to keep as fixtures.
 KNOWN = {
     "cop_hat300-1.txt": 292,
     "cop_hat300-2.txt": 275,
@@ -42,6 +43,11 @@ KNOWN = {
     "cop_hat700-1.txt": 689,
     "cop_hat1000-1.txt": 990,
     "Sister-Cities.txt": 5527,
+    "formated_maayan-figeys.txt": 316,
+    "formated_maayan-vidal.txt": 1077,
+    "formated_movielens-100k_rating.txt": 849,
+    "formated_wikipedia_link_csb.txt": 1998,
+    "lastfm-asia.txt": 3447,
 }
 # Shipped with an answer in the same table but too hard for the CPU oracle to
 # confirm in minutes; kept as data for GPU runs and marked "cpu_confirmed": false.
@@ -49,7 +55,25 @@ KNOWN_HARD = {
     "cop_hat300-3.txt": 264,
     "cop_hat500-2.txt": 464,
     "Power-Grid.txt": 2203,
+    "formated_wikipedia_link_lo.txt": 1536,
 }
+# Files whose vertices are numbered from 0 (the reference's loader,
+# src/auxFunctions.cpp:151-153, expects 1 and cannot read them).
+ZERO_INDEXED = {
+    "formated_maayan-figeys.txt", "formated_maayan-vidal.txt", "formated_movielens-100k_rating.txt",
+    "formated_wikipedia_link_csb.txt", "formated_wikipedia_link_lo.txt", "lastfm-asia.txt",
+}
+
+
+def load_reference_graph(path, index_base):
+    """A reference graph file as an oracle Graph: the oracle's loader for the
+    reference's 1-indexed format, a plain token parse for 0-indexed files."""
+    if index_base == 1:
+        return O.load(path)
+    tok = np.array(open(path).read().split(), dtype=np.int64)
+    n, m = int(tok[0]), int(tok[1])
+    edges = tok[2:2 + 2 * m].reshape(m, 2) - index_base
+    return O.from_edges(n, edges)
 
 
 def pack_graph(n, edges):
@@ -87,7 +111,8 @@ def reference_fixtures():
     os.makedirs(os.path.join(HERE, "graphs"), exist_ok=True)
     for fname, mvc in list(KNOWN.items()) + list(KNOWN_HARD.items()):
         path = os.path.join(REF_GRAPHS, fname)
-        g = O.load(path)
+        base = 0 if fname in ZERO_INDEXED else 1
+        g = load_reference_graph(path, base)
         assert O.check_graph(g), fname
         src = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(g.srcPtr.astype(np.int64)))
         keep = src < g.dst
@@ -98,7 +123,7 @@ def reference_fixtures():
             f.write(blob)
         n2, e2 = unpack_graph(os.path.join(HERE, "graphs", name + ".npz"))
         assert n2 == g.n and graphgen.edge_checksum(e2) == graphgen.edge_checksum(edges)
-        entry = {"file": f"test/test_graphs/{fname}", "n": int(g.n), "m": int(g.m), "mvc": mvc,
+        entry = {"file": f"test/test_graphs/{fname}", "index_base": base, "n": int(g.n), "m": int(g.m), "mvc": mvc,
                  "source": "test/runGraphs.py:8-30", "edge_sha256": graphgen.edge_checksum(edges),
                  "cpu_confirmed": fname in KNOWN}
         if fname in KNOWN:
@@ -192,6 +217,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--with-slow", action="store_true")
     ap.add_argument("--skip-known", action="store_true")
+    ap.add_argument("--only-known", action="store_true")
     args = ap.parse_args()
     if not args.skip_known:
         if not os.path.isdir(REF_GRAPHS):
@@ -199,6 +225,8 @@ def main():
         known = reference_fixtures()
         with open(os.path.join(HERE, "known_answers.json"), "w") as f:
             json.dump(known, f, indent=1, sort_keys=True)
+    if args.only_known:
+        return
     vec_path = os.path.join(HERE, "oracle_vectors.json")
     old = {}
     if os.path.exists(vec_path):

@@ -43,6 +43,30 @@ def test_cli_upper_bounds_are_the_references(tmp_path):
     assert "Graph file: g.txt" in r.stdout
 
 
+def _zero_indexed_fixture(tmp_path, name):
+    """A 0-indexed reference graph, written back in its file layout from the
+    committed fixture."""
+    from golden.make_golden import unpack_graph
+    import json
+    meta = json.load(open(os.path.join(HERE, "golden", "known_answers.json")))[name]
+    n, e = unpack_graph(os.path.join(HERE, "golden", "graphs", name + ".npz"))
+    path = tmp_path / (name + ".txt")
+    path.write_text(f"{n} {len(e)}\n" + "".join(f"{u}\t{v}\n" for u, v in e))
+    return path, n, e, meta
+
+
+def test_cli_reads_zero_indexed_files_with_z(tmp_path):
+    """-z reads a file whose vertices start at 0 (the reference ships six with
+    answers in test/runGraphs.py:21-29); without it the file is refused."""
+    path, n, e, meta = _zero_indexed_fixture(tmp_path, "formated_maayan-figeys")
+    r = run_vc(["-f", path.name, "-z", "-v", "sequential"], tmp_path)
+    assert r.returncode == 2 and "not part of this GPU build" in r.stderr
+    got_max = int(re.search(r"Approximate Remove Max Minimum is: (\d+)", r.stdout).group(1))
+    assert got_max == O.remove_max(O.from_edges(n, e))
+    r = run_vc(["-f", path.name], tmp_path)
+    assert r.returncode == 2 and "cannot load" in r.stderr
+
+
 def test_cli_rejects_bad_input(tmp_path):
     path, _, _ = _graph(tmp_path)
     assert "Unrecognized option" in run_vc(["-x"], tmp_path).stderr
@@ -74,3 +98,13 @@ def test_cli_solves_and_appends_the_results_row(tmp_path):
     assert rows[0][0] == "run1" and rows[0][4] == "mvc" and int(rows[0][18]) == opt
     assert rows[1][4] == "pvc" and rows[1][5] == "stackonly" and int(rows[1][20]) == 0
     assert int(rows[2][20]) == 1
+
+
+@pytest.mark.gpu
+def test_cli_solves_a_zero_indexed_reference_file(tmp_path):
+    """The reference's MVC for formated_maayan-vidal (test/runGraphs.py:28)
+    through `vc -z`."""
+    path, n, e, meta = _zero_indexed_fixture(tmp_path, "formated_maayan-vidal")
+    r = run_vc(["-f", path.name, "-z"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert int(re.search(r"Size of minimum vertex cover: (\d+)", r.stdout).group(1)) == meta["mvc"] == 1077

@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
 ROOT = os.path.dirname(HERE)
 
-from golden.make_golden import make_graph, unpack_graph  # noqa: E402
+from golden.make_golden import load_reference_graph, make_graph, unpack_graph  # noqa: E402
 
 
 def brute_mvc(n, edges):
@@ -85,7 +85,7 @@ def test_known_answer_files_match_reference_copies():
     if not os.path.isdir(base):
         pytest.skip("reference not mounted")
     for name, meta in KNOWN.items():
-        g = O.load(os.path.join(base, meta["file"]))
+        g = load_reference_graph(os.path.join(base, meta["file"]), meta["index_base"])
         n, e = unpack_graph(os.path.join(GOLD, "graphs", name + ".npz"))
         g2 = O.from_edges(n, e)
         assert g.n == g2.n and g.m == g2.m
@@ -384,6 +384,40 @@ def test_product_loader_reads_what_the_reference_reads(tmp_path):
     short.write_text("\n".join([f"{n} {len(e) + 3}"] + [f"{u + 1} {v + 1}" for u, v in e]) + "\n")
     with pytest.raises(vcgpu.VCError):
         vcgpu.load_graph_file(str(short))
+
+
+@pytest.mark.parametrize("name", sorted(k for k, v in KNOWN.items() if v["index_base"] == 0))
+def test_product_loads_zero_indexed_reference_files(name, tmp_path):
+    """vc_load_graph_file_indexed(path, 0, ...) reads the reference's
+    0-indexed files (here rewritten from the fixture in their layout: header,
+    then one edge a line, tab-separated) into the CSR the oracle builds from
+    the same edges; the 1-indexed loader refuses them (vertex 0)."""
+    vcgpu = _vc()
+    meta = KNOWN[name]
+    n, e = unpack_graph(os.path.join(GOLD, "graphs", name + ".npz"))
+    p = tmp_path / (name + ".txt")
+    p.write_text(f"{n} {len(e)}\n" + "".join(f"{u}\t{v}\n" for u, v in e))
+    a = vcgpu.load_graph_file(str(p), index_base=0)
+    b = O.from_edges(n, e)
+    assert (a.n, a.m) == (meta["n"], meta["m"])
+    assert np.array_equal(a.srcPtr, b.srcPtr) and np.array_equal(a.dst, b.dst)
+    assert np.array_equal(a.degree, b.degree) and vcgpu.check_graph(a)
+    if int(e.min()) == 0:
+        with pytest.raises(vcgpu.VCError):
+            vcgpu.load_graph_file(str(p))
+
+
+def test_product_index_base_errors(tmp_path):
+    vcgpu = _vc()
+    p = tmp_path / "z.txt"
+    p.write_text("3 2\n0 1\n1 2\n")
+    g = vcgpu.load_graph_file(str(p), index_base=0)
+    assert (g.n, g.m) == (3, 2) and list(g.degree) == [1, 2, 1]
+    p.write_text("3 2\n0 1\n1 3\n")  # vertex 3 is out of range when counting from 0
+    with pytest.raises(vcgpu.VCError):
+        vcgpu.load_graph_file(str(p), index_base=0)
+    with pytest.raises(vcgpu.VCError):
+        vcgpu.load_graph_file(str(p), index_base=2)
 
 
 @pytest.mark.parametrize("vec", VECS[:40], ids=lambda v: v["key"])
