@@ -246,6 +246,55 @@ struct Node {
         else
             return s1w[v];
     }
+    // Degree of v (-1 for v >= n) and s1(v), read at a valid index without a
+    // branch; one read in the packed layout.
+    __device__ __forceinline__ int deg_s1(uint32_t v, uint32_t n, uint32_t &s) const
+    {
+        const uint32_t i = v < n ? v : 0u;
+        int d;
+        if constexpr (kPacked) {
+            const uint32_t x = dw[i];
+            d = (int)(int16_t)(uint16_t)x;
+            s = x >> 16;
+        } else {
+            d = deg32[i];
+            s = s1w[i];
+        }
+        return v < n ? d : -1;
+    }
+    // Degrees and s2 of the vertex pair (v, v + 1), v even, -1 past n: one
+    // 8-byte read per array (16 bytes for 64-bit s2), issued together, so a
+    // scan waits once per pair.  An even v < n <= npad has v + 1 < npad, and
+    // v >= n reads at 0, so every read stays inside the arrays.
+    __device__ __forceinline__ void pair(uint32_t v, uint32_t n, int &d0, int &d1) const
+    {
+        const uint32_t i = v < n ? v : 0u;
+        int a, b;
+        if constexpr (kPacked) {
+            const uint2 x = *reinterpret_cast<const uint2 *>(dw + i);
+            a = (int)(int16_t)(uint16_t)x.x;
+            b = (int)(int16_t)(uint16_t)x.y;
+        } else {
+            const int2 x = *reinterpret_cast<const int2 *>(deg32 + i);
+            a = x.x;
+            b = x.y;
+        }
+        d0 = v < n ? a : -1;
+        d1 = v + 1u < n ? b : -1;
+    }
+    __device__ __forceinline__ void pair_s2(uint32_t v, uint32_t n, s2_t &q0, s2_t &q1) const
+    {
+        const uint32_t i = v < n ? v : 0u;
+        if constexpr (kPacked) {
+            const uint2 x = *reinterpret_cast<const uint2 *>(s2 + i);
+            q0 = x.x;
+            q1 = x.y;
+        } else {
+            const ulonglong2 x = *reinterpret_cast<const ulonglong2 *>(s2 + i);
+            q0 = x.x;
+            q1 = x.y;
+        }
+    }
     // degree := -1 (v joins the cover); its sketch is not read again
     __device__ __forceinline__ void remove(uint32_t v) const
     {
@@ -488,12 +537,13 @@ __device__ __forceinline__ void lanes_add(Lanes &st, int dv, uint32_t v)
 // for roots (full = true) and after a pass whose marks hit the list capacity
 // (the unmarked rest is found by the next full pass, as in the reference).
 template <class N>
-__device__ __forceinline__ uint32_t leaf_mark(const N &w, uint32_t v, int dv)
+__device__ __forceinline__ uint32_t leaf_mark(const N &w, uint32_t v, uint32_t n)
 {
     // branch-free: every lane reads at a valid index, the result is selected
-    const bool leaf = dv == 1;  // false for padding lanes (dv == -1)
-    const uint32_t u = w.s1(leaf ? v : 0u);
-    const int du = deg_or_none(w, u, w.n);
+    uint32_t s;
+    const bool leaf = w.deg_s1(v, n, s) == 1;  // false past n and for empty candidates
+    const uint32_t u = leaf ? s : 0u;
+    const int du = deg_or_none(w, u, n);
     const bool mark = leaf & ((du > 1) | ((du == 1) & (u < v)));
     return mark ? u : kNone;
 }
@@ -505,17 +555,13 @@ __device__ uint32_t rule_leaf(const N &w, uint32_t &nc, bool &full)
     for (;;) {
         uint32_t L = 0;
         if (full || nc > w.ncap) {
-            for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
-                const uint32_t v = base + lane_id();
-                const int dv = deg_or_none(w, v, nv);
-                L = append(w, L, leaf_mark(w, v, dv), true);
-            }
+            for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave)
+                L = append(w, L, leaf_mark(w, base + lane_id(), nv), true);
         } else {
+            const uint32_t nv = uni(w.n);
             for (uint32_t i0 = 0; i0 < nc; i0 += kWave) {
                 const uint32_t i = i0 + lane_id();
-                const uint32_t v = i < nc ? (uint32_t)w.cand[i] : kNone;
-                const int dv = deg_or_none(w, v, w.n);
-                L = append(w, L, leaf_mark(w, v, dv), true);
+                L = append(w, L, leaf_mark(w, i < nc ? (uint32_t)w.cand[i] : kNone, nv), true);
             }
         }
         nc = 0;  // consumed; the apply below lists the next candidates
@@ -550,6 +596,28 @@ __device__ __forceinline__ bool adj_has(const uint32_t *dst, uint32_t lo, uint32
 // vertex whose two live neighbours are adjacent puts both neighbours in the
 // cover.  Skipped when a neighbour has degree 1, or degree 2 with a lower
 // index, as in the reference.  All triangles of a pass are applied together.
+// One vertex of a triangle scan: v of degree dv with sketch sum q.  Sets
+// c1, c2 to the two neighbours to cover, or settles v.
+template <class N>
+__device__ __forceinline__ void triangle_at(const N &w, uint32_t v, int dv, typename N::s2_t q, uint32_t &c1,
+                                            uint32_t &c2)
+{
+    using S2 = typename N::s2_t;
+    uint32_t n1, n2;
+    if (dv == 2 && (q & kSettled<S2>) == 0 && two_neighbours(w, v, q, n1, n2)) {
+        // skipped when a neighbour has degree 1, or degree 2 and a
+        // lower index (src/helperFunctions.cuh:379-385)
+        const int d1 = w.deg(n1), d2 = w.deg(n2);
+        const bool skip = (d1 == 1) | ((d1 == 2) & (n1 < v)) | (d2 == 1) | ((d2 == 2) & (n2 < v));
+        if (!skip && adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
+            c1 = n1;
+            c2 = n2;
+        } else {
+            w.s2[v] = q | kSettled<S2>;
+        }
+    }
+}
+
 template <class N>
 __device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
 {
@@ -557,29 +625,25 @@ __device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
     for (;;) {
         uint32_t L = 0;
         lanes_reset(st);
-        for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
-            uint32_t v = base + lane_id();
-            uint32_t c1 = kNone, c2 = kNone;
-            int dv = deg_or_none(w, v, nv);
-            lanes_add(st, dv, v);
-            using S2 = typename N::s2_t;
-            uint32_t n1, n2;
-            S2 q = 0;
-            if (dv == 2) q = w.s2[v];
-            if (dv == 2 && (q & kSettled<S2>) == 0 && two_neighbours(w, v, q, n1, n2)) {
-                // skipped when a neighbour has degree 1, or degree 2 and a
-                // lower index (src/helperFunctions.cuh:379-385)
-                const int d1 = w.deg(n1), d2 = w.deg(n2);
-                const bool skip = (d1 == 1) | ((d1 == 2) & (n1 < v)) | (d2 == 1) | ((d2 == 2) & (n2 < v));
-                if (!skip && adj_has(w.dst, w.src[n2], w.src[n2 + 1], n1)) {
-                    c1 = n1;
-                    c2 = n2;
-                } else {
-                    w.s2[v] = q | kSettled<S2>;
-                }
+        // two vertices per lane: (v0, v0 + 1), so a lane's vertices still
+        // come in increasing order
+        for (uint32_t base = 0, nv = uni(w.n); base < nv; base += 2u * kWave) {
+            const uint32_t v0 = base + 2u * lane_id();
+            int d0, d1;
+            typename N::s2_t q0, q1;
+            w.pair(v0, nv, d0, d1);
+            w.pair_s2(v0, nv, q0, q1);
+            lanes_add(st, d0, v0);
+            lanes_add(st, d1, v0 + 1u);
+            uint32_t a1 = kNone, a2 = kNone, b1 = kNone, b2 = kNone;
+            triangle_at(w, v0, d0, q0, a1, a2);
+            triangle_at(w, v0 + 1u, d1, q1, b1, b2);
+            if (__ballot((a1 != kNone) | (b1 != kNone)) != 0ull) {
+                L = append(w, L, a1, true);
+                L = append(w, L, a2, true);
+                L = append(w, L, b1, true);
+                L = append(w, L, b2, true);
             }
-            L = append(w, L, c1, true);
-            L = append(w, L, c2, true);
         }
         if (L == 0) break;
         apply_list(w, L, nc);
@@ -603,13 +667,20 @@ __device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st, uin
         uint32_t L = 0;
         int rest = 0;  // largest degree left unmarked by this pass (per lane)
         lanes_reset(st);
-        for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
-            const uint32_t v = base + lane_id();
-            const int dv = deg_or_none(w, v, nv);  // -1 past n: adds nothing below
-            lanes_add(st, dv, v);
-            const bool hit = (dv > 0) & ((uint32_t)dv + d + total >= T);
-            rest = hit ? rest : max(rest, dv);
-            L = append(w, L, hit ? v : kNone, false);
+        for (uint32_t base = 0, nv = uni(w.n); base < nv; base += 2u * kWave) {
+            const uint32_t v0 = base + 2u * lane_id();
+            int d0, d1;
+            w.pair(v0, nv, d0, d1);  // -1 past n: adds nothing below
+            lanes_add(st, d0, v0);
+            lanes_add(st, d1, v0 + 1u);
+            const bool h0 = (d0 > 0) & ((uint32_t)d0 + d + total >= T);
+            const bool h1 = (d1 > 0) & ((uint32_t)d1 + d + total >= T);
+            rest = h0 ? rest : max(rest, d0);
+            rest = h1 ? rest : max(rest, d1);
+            if (__ballot(h0 | h1) != 0ull) {
+                L = append(w, L, h0 ? v0 : kNone, false);
+                L = append(w, L, h1 ? v0 + 1u : kNone, false);
+            }
         }
         if (L == 0) break;
         apply_list(w, L, nc);
