@@ -122,6 +122,21 @@ __device__ __forceinline__ int wave_incl_scan(int x)
     return v;
 }
 
+// Inclusive prefix maximum over the 64 lanes of non-negative values: DPP
+// row_shr 1/2/4/8 (lanes shifted in from outside the row read 0), then
+// row_bcast 15/31 across rows.
+__device__ __forceinline__ int wave_incl_max(int x)
+{
+    int v = x;
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, true));
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // HBM atomics (agent scope)
 // ---------------------------------------------------------------------------
@@ -179,7 +194,7 @@ __device__ __forceinline__ bool past_deadline(const SolveArgs &a)
 // ---------------------------------------------------------------------------
 
 // Neighbour sketch of a tree node.  Beside its degree (number of live
-// neighbours; -1 = in the cover) every vertex v keeps
+// neighbours; negative = in the cover) every vertex v keeps
 //   s1(v) = sum of the ids of its live neighbours
 //   s2(v) = sum of the squared ids of its live neighbours
 // so a degree-1 vertex's neighbour is s1(v), and a degree-2 vertex's two
@@ -224,7 +239,7 @@ struct Node {
     uint32_t ncap;
     uint32_t *mark;      // [mwords] bitset, all zero between passes
     uint32_t *list;      // [lcap]: vertices marked in the current pass
-    int *xs;             // [3 * 64] scratch of apply_list: edge offsets, list bases, list vertices
+    int *xs;             // [64] scratch of apply_list: the owner of each edge of a window
     const uint32_t *src; // graph srcPtr (HBM)
     const uint32_t *dst; // graph dst (HBM)
     uint32_t n;
@@ -295,7 +310,10 @@ struct Node {
             q1 = x.y;
         }
     }
-    // degree := -1 (v joins the cover); its sketch is not read again
+    // degree := -1 (v joins the cover).  Its sketch is not read again, and
+    // later decrements through drop() only make the degree more negative
+    // (at most its original degree of them: an int16 degree stays below 0,
+    // so no borrow reaches s1).
     __device__ __forceinline__ void remove(uint32_t v) const
     {
         if constexpr (kPacked)
@@ -303,8 +321,9 @@ struct Node {
         else
             deg32[v] = -1;
     }
-    // Live vertex t loses its live neighbour u; several lanes may hit the
-    // same t.  Returns t's degree before the update.
+    // Vertex t loses its neighbour u; several lanes may hit the same t.
+    // Returns t's degree before the update: negative when t is already in
+    // the cover (the update then changes nothing that is read again).
     __device__ __forceinline__ int drop(uint32_t t, uint32_t u) const
     {
         int old;
@@ -443,9 +462,10 @@ __device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, typename 
 // of them get -1, then each decrements its neighbours that are still live
 // (src/helperFunctions.cuh:270-289).  Listed vertices are distinct and live.
 // The adjacency lists of up to 64 listed vertices are laid end to end over
-// the 64 lanes (wave prefix sum of their lengths; each lane finds the list
-// entry that owns its edge with a 6-step shuffle search), so one round of
-// global loads serves 64 edges whatever the degrees.
+// the 64 lanes (wave prefix sum of their lengths), so one round of global
+// loads serves 64 edges whatever the degrees.  Each edge is decremented
+// without a prior degree read: drop() returns the old degree, and a
+// negative one (a neighbour already in the cover) is ignored.
 template <class N>
 __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
 {
@@ -454,9 +474,7 @@ __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
     for (uint32_t i = lane_id(); i < L; i += kWave) w.remove(w.list[i]);
     w.sync();
     const int lane = (int)lane_id();
-    int *const xoff = w.xs;              // exclusive edge offset of list entry j
-    int *const xbase = w.xs + kWave;     // first adjacency index of list entry j
-    int *const xvert = w.xs + 2 * kWave; // list entry j
+    int *const slot = w.xs;  // [64]: owner of each edge of one 64-edge window
     for (uint32_t c0 = 0; c0 < L; c0 += kWave) {
         const uint32_t cnt = min(L - c0, (uint32_t)kWave);
         int b = 0, len = 0, u = 0;
@@ -466,32 +484,45 @@ __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
             len = (int)w.src[u + 1] - b;
         }
         const int incl = wave_incl_scan(len);
-        xoff[lane] = incl - len;
-        xbase[lane] = b;
-        xvert[lane] = u;
+        const int excl = incl - len;
         const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
-        w.sync();
+        if (cnt == 1u) {
+            // one listed vertex (most leaf passes): its own edges, no owner search
+            const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane(b, 0);
+            const uint32_t u0 = (uint32_t)__builtin_amdgcn_readlane(u, 0);
+#pragma unroll 1
+            for (int x0 = 0; x0 < total; x0 += kWave) {
+                const int x = x0 + lane;
+                uint32_t leaf = kNone;
+                if (x < total) {
+                    const uint32_t t = w.dst[b0 + (uint32_t)x];
+                    if (w.drop(t, u0) == 2) leaf = t;
+                }
+                nc = add_leaf_cands(w, nc, leaf);
+            }
+            continue;
+        }
 #pragma unroll 1
         for (int x0 = 0; x0 < total; x0 += kWave) {
+            // owner of edge x = the last entry j with excl_j <= x.  Entry j0
+            // owns x0; each later entry that starts inside the window writes
+            // its index at its start, and a prefix max over the lanes carries
+            // every owner to the edges after its start.
+            const int j0 = 63 - __clzll(__ballot((uint32_t)lane < cnt && excl <= x0));
+            slot[lane] = j0;
+            if ((uint32_t)lane < cnt && len > 0 && excl > x0 && excl < x0 + kWave) slot[excl - x0] = lane;
+            w.sync();
+            const int own = wave_incl_max(slot[lane]);
+            const int ob = __shfl(b, own), ox = __shfl(excl, own), ou = __shfl(u, own);
             const int x = x0 + lane;
             uint32_t leaf = kNone;
             if (x < total) {
-                // owner: last entry j < cnt with xoff[j] <= x (a leaf pass
-                // usually applies one vertex: no search then)
-                int pos = 0;
-                if (cnt > 1u) {
-#pragma unroll
-                    for (int step = 32; step >= 1; step >>= 1) {
-                        const int cand = pos + step;
-                        if (cand < (int)cnt && xoff[cand] <= x) pos = cand;
-                    }
-                }
-                const uint32_t t = w.dst[xbase[pos] + (x - xoff[pos])];
-                if (w.deg(t) != -1 && w.drop(t, (uint32_t)xvert[pos]) == 2) leaf = t;
+                const uint32_t t = w.dst[(uint32_t)(ob + (x - ox))];
+                if (w.drop(t, (uint32_t)ou) == 2) leaf = t;
             }
             nc = add_leaf_cands(w, nc, leaf);
+            w.sync();
         }
-        w.sync();
     }
     for (uint32_t i = lane_id(); i < L; i += kWave) w.mark[w.list[i] >> 5] = 0u;
     w.sync();
@@ -751,7 +782,7 @@ __device__ void take_vertex(const N &w, uint32_t v, uint32_t &nc)
         if (x < e) {
             const uint32_t t = w.dst[x];
             const int dt = w.deg(t);
-            if (dt != -1) {
+            if (dt >= 0) {
                 // the neighbours of one vertex are distinct: plain updates
                 w.drop_plain(t, v, dt);
                 if (dt == 2) leaf = t;
@@ -778,7 +809,7 @@ __device__ uint32_t take_neighbours(const N &w, uint32_t v, uint32_t &nc)
             uint32_t cand = kNone;
             if (x < e) {
                 uint32_t t = w.dst[x];
-                if (w.deg(t) != -1) cand = t;
+                if (w.deg(t) >= 0) cand = t;
             }
             L = append(w, L, cand, false);
         }
@@ -816,8 +847,11 @@ __device__ void init_root(const N &w, const int32_t *deg0, uint32_t npad)
 // Copies the node state of the first cnt vertices between two node-state
 // layouts with the same npad (node storage <-> a stack or worklist slot).
 // A slot is sized for the largest graph of the launch; only the current
-// graph's vertices travel (rounded up to whole 16-byte vectors), and all
-// loads of a lane are issued before its stores.
+// graph's vertices travel (rounded up to whole 16-byte vectors).  The packed
+// layout's two arrays are copied as one run of 16-byte vectors over both
+// (the second array starts npad * 4 bytes in, npad a multiple of 32): one
+// vector in flight per lane keeps the copy within the kernel's register
+// budget.
 template <class N>
 __device__ __forceinline__ void copy_state(char *__restrict__ dst, const char *__restrict__ src, uint32_t npad,
                                            uint32_t cnt)
@@ -827,10 +861,10 @@ __device__ __forceinline__ void copy_state(char *__restrict__ dst, const char *_
     int4 *d4 = reinterpret_cast<int4 *>(dst);
     const uint32_t o1 = npad >> 2;       // second array, at npad * 4 bytes
     if constexpr (N::kPacked) {
-        for (uint32_t i = lane_id(); i < q; i += kWave) {
-            const int4 x = s4[i], y = s4[o1 + i];
-            d4[i] = x;
-            d4[o1 + i] = y;
+        // vector k < q is entry k of the first array, k >= q entry k - q of the second
+        for (uint32_t k = lane_id(); k < 2u * q; k += kWave) {
+            const uint32_t i = k < q ? k : o1 + (k - q);
+            d4[i] = s4[i];
         }
     } else {
         const uint32_t o2 = npad >> 1;   // 8-byte s2 entries, at npad * 8 bytes
@@ -950,7 +984,7 @@ __device__ void write_cover(const SolveArgs &a, const N &w, const GDesc &gd)
     uint32_t *cw = a.covers + gd.off_cover;
     for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave) {
         uint32_t v = base + lane_id();
-        uint64_t m = __ballot(v < w.n && w.deg(v) == -1);
+        uint64_t m = __ballot(v < w.n && w.deg(v) < 0);
         if (lane_id() == 0) {
             cw[base / 32] = (uint32_t)m;
             cw[base / 32 + 1] = (uint32_t)(m >> 32);
@@ -1232,8 +1266,11 @@ __device__ __forceinline__ GDesc load_desc(const GDesc *p)
 
 constexpr uint32_t kFlushEvery = 1024;
 
+// The kernel body takes the launch arguments by reference from a one-line
+// kernel: hipcc then allocates 55 instead of 81 VGPRs for the int16 LDS
+// variant (same source otherwise), so the LDS budget alone sets occupancy.
 template <typename DegT, bool HBM, bool SERVE>
-__global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
+__device__ __forceinline__ void solve_body(const SolveArgs &a)
 {
     extern __shared__ int4 smem4[];
     const uint32_t wib = threadIdx.x / kWave;
@@ -1241,7 +1278,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
     if (wid >= a.workers) return;
 
     // node storage: node state (npad * node_bytes_per_vertex) | list [lcap u32] | mark [mwords u32] |
-    // xs [192 i32] | cand [npad]; a stack or worklist slot holds the node state
+    // xs [64 i32] | cand [npad]; a stack or worklist slot holds the node state
     char *store = HBM ? a.hbm_nodes + (size_t)wid * a.worker_bytes
                       : reinterpret_cast<char *>(smem4) + (size_t)wib * a.worker_bytes;
     const uint32_t slot_bytes = a.npad * node_bytes_per_vertex<DegT>();
@@ -1262,7 +1299,7 @@ __global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
     w.list = reinterpret_cast<uint32_t *>(store + slot_bytes);
     w.mark = w.list + a.lcap;
     w.xs = reinterpret_cast<int *>(w.mark + a.mwords);
-    w.cand = reinterpret_cast<typename NodeT::cand_t *>(w.xs + 3 * kWave);
+    w.cand = reinterpret_cast<typename NodeT::cand_t *>(w.xs + kWave);
     w.ncap = a.npad;
     w.src = nullptr;
     w.dst = nullptr;
@@ -1503,6 +1540,12 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
     }
 }
 
+template <typename DegT, bool HBM, bool SERVE>
+__global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
+{
+    solve_body<DegT, HBM, SERVE>(a);
+}
+
 template __global__ void solve_kernel<int16_t, false, false>(SolveArgs);
 template __global__ void solve_kernel<int16_t, true, false>(SolveArgs);
 template __global__ void solve_kernel<int32_t, true, false>(SolveArgs);
@@ -1695,7 +1738,7 @@ struct Plan {
 };
 
 // Per-worker node storage for a degree type: deg [npad] | s1 [npad] | s2 [npad] | list [lcap] |
-// mark [mwords] | xs [192] | cand [npad].  A stack / worklist slot is deg | s1 | s2.
+// mark [mwords] | xs [64] | cand [npad].  A stack / worklist slot is deg | s1 | s2.
 void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes)
 {
     pl.deg_bytes = deg_bytes;
@@ -1706,7 +1749,7 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes)
     pl.mwords = round_up((nmax + 31u) / 32u, 4u);
     const uint32_t cand_bytes = pl.npad * (deg_bytes == 2u ? 2u : 4u);
     pl.worker_bytes =
-        round_up(pl.slot_bytes + 4u * pl.lcap + 4u * pl.mwords + 4u * 3u * vcg::kWave + cand_bytes, 64u);
+        round_up(pl.slot_bytes + 4u * pl.lcap + 4u * pl.mwords + 4u * vcg::kWave + cand_bytes, 64u);
 }
 
 // Launch shape shared by the batch solve and the server: node storage,
