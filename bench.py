@@ -346,7 +346,7 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=1000.0, help="served: Poisson arrival rate (requests/s)")
     ap.add_argument("--small-blocks", type=int, default=0,
-                    help="served: workgroups of the small-graph server (0 = occupancy x CUs)")
+                    help="served: workgroups of the small-graph server (0 = the library default, 2 workers per CU)")
     ap.add_argument("--large-blocks", type=int, default=32,
                     help="served: one-wave workgroups of the large-graph server (each holds an exact-depth stack, "
                          "about 2 GB for BA n=20000)")

@@ -185,7 +185,10 @@ typedef struct {
     uint32_t max_vertices;  /* largest graph the server admits (sizes LDS and stacks) */
     uint32_t slots;         /* requests on the device at once; 0 = 4096 */
     uint64_t arena_bytes;   /* HBM for the graphs + covers in flight; 0 = 1 GiB */
-    uint32_t num_blocks;    /* workgroups of the persistent kernel; 0 = occupancy x CUs */
+    uint32_t num_blocks;    /* workgroups of the persistent kernel; 0 = 2 workers per CU
+                               (at most the occupancy): idle waves poll, so a
+                               small grid serves a request stream faster and
+                               leaves CUs to other kernels */
     uint32_t block_dim;     /* threads per workgroup (multiple of 64, <= 256); 0 = auto */
     uint32_t stack_depth;   /* stack entries per worker; 0 = max_vertices + 1 */
     uint32_t time_limit_ms; /* lifetime limit of the server kernel; 0 = one hour */

@@ -82,6 +82,8 @@ struct HostCtrl {
     uint32_t pad0[15];
     uint32_t shutdown;  // 1 = exit once every admitted request is done
     uint32_t pad1[15];
+    uint32_t error;     // the kernel's glob->error, published as its workers exit
+    uint32_t pad2[15];
 };
 
 struct Completion {
@@ -123,7 +125,7 @@ struct SolveArgs {
     void *q_deg;            // [qsize][npad] DegT
     uint4 *q_meta;          // [qsize] = (graph, d, branch vertex, 0)
     uint32_t *tickets;      // [qsize]
-    WorkerStats *wstats;    // [workers]
+    WorkerStats *wstats;    // [workers] (pinned host memory for a server)
     uint32_t num_graphs;
     uint32_t npad;          // slot length in DegT values
     uint32_t lcap;          // mark-list capacity per worker (multiple of 4)

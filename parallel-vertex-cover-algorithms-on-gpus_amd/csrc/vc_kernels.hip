@@ -238,7 +238,7 @@ struct Node {
     cand_t *cand;        // [ncap]: vertices whose degree fell to 1 since the last leaf pass
     uint32_t ncap;
     uint32_t *mark;      // [mwords] bitset, all zero between passes
-    uint32_t *list;      // [lcap]: vertices marked in the current pass
+    cand_t *list;        // [lcap]: vertices marked in the current pass
     int *xs;             // [64] scratch of apply_list: the owner of each edge of a window
     const uint32_t *src; // graph srcPtr (HBM)
     const uint32_t *dst; // graph dst (HBM)
@@ -382,7 +382,7 @@ __device__ __forceinline__ uint32_t append(const N &w, uint32_t L, uint32_t cand
     if (keep) {
         uint32_t pos = L + lanes_below(m);
         if (pos < w.lcap)
-            w.list[pos] = cand;
+            w.list[pos] = (typename N::cand_t)cand;
         else if (dedupe)
             __hip_atomic_fetch_and(&w.mark[cand >> 5], ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
@@ -1296,8 +1296,8 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         w.s1w = reinterpret_cast<uint32_t *>(store + (size_t)a.npad * 4u);
         w.s2 = reinterpret_cast<typename NodeT::s2_t *>(store + (size_t)a.npad * 8u);
     }
-    w.list = reinterpret_cast<uint32_t *>(store + slot_bytes);
-    w.mark = w.list + a.lcap;
+    w.list = reinterpret_cast<typename NodeT::cand_t *>(store + slot_bytes);
+    w.mark = reinterpret_cast<uint32_t *>(w.list + a.lcap);  // lcap is a multiple of 4
     w.xs = reinterpret_cast<int *>(w.mark + a.mwords);
     w.cand = reinterpret_cast<typename NodeT::cand_t *>(w.xs + kWave);
     w.ncap = a.npad;
@@ -1515,12 +1515,37 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         ws.donations = donations;
         ws.steals = steals;
         a.wstats[wid] = ws;
+        if constexpr (SERVE) {
+            // a server's statistics and error word live in pinned host
+            // memory: stopping a server then needs no copy, which could wait
+            // for another persistent kernel on the device
+            const uint32_t e = ld_rlx(&a.glob->error);
+            if (e) __hip_atomic_store(&a.host->error, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
 // Per-launch initialisation with plain stores (no memset nodes, so the whole
 // solve can be captured in a HIP graph).  units = outstanding subtrees per
 // graph at start: 1 root (hybrid) or 2^start_depth paths (static split).
+// Results of a solve, copied by a kernel into mapped pinned host memory.
+// hipMemcpyAsync would do the same, but such a copy has been measured to
+// wait until a server's persistent kernel on the device ends, while a kernel
+// on the same stream is scheduled beside it like the solve itself.
+struct Publish {
+    const uint32_t *src[4];
+    uint32_t *dst[4];
+    uint32_t words[4];
+};
+
+__global__ void publish_kernel(Publish p)
+{
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (int k = 0; k < 4; ++k)
+        for (uint32_t i = tid; i < p.words[k]; i += stride) p.dst[k][i] = p.src[k][i];
+}
+
 __global__ void init_kernel(SolveArgs a, uint32_t units)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1746,14 +1771,17 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes)
     pl.slot_bytes = pl.npad * (deg_bytes == 2u ? vcg::node_bytes_per_vertex<int16_t>()
                                                : vcg::node_bytes_per_vertex<int32_t>());
     pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), 1024u);
+    const uint32_t id_bytes = deg_bytes == 2u ? 2u : 4u;  // mark and candidate list entries
     pl.mwords = round_up((nmax + 31u) / 32u, 4u);
-    const uint32_t cand_bytes = pl.npad * (deg_bytes == 2u ? 2u : 4u);
+    const uint32_t cand_bytes = pl.npad * id_bytes;
     pl.worker_bytes =
-        round_up(pl.slot_bytes + 4u * pl.lcap + 4u * pl.mwords + 4u * vcg::kWave + cand_bytes, 64u);
+        round_up(pl.slot_bytes + id_bytes * pl.lcap + 4u * pl.mwords + 4u * vcg::kWave + cand_bytes, 64u);
 }
 
 // Launch shape shared by the batch solve and the server: node storage,
 // workgroup, grid from the occupancy query, worklist, and the memory plan.
+constexpr uint32_t kServeWorkersPerCU = 2;
+
 vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_params *p, Plan &pl, bool serve)
 {
     const uint32_t version = p->version;
@@ -1788,7 +1816,18 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
         return VC_EDEVICE;
     if (bpc < 1) return VC_ENOTSUP;
     pl.blocks_per_cu = (uint32_t)bpc;
-    pl.num_blocks = p->num_blocks ? p->num_blocks : (uint32_t)bpc * (uint32_t)ctx->cu_count;
+    uint32_t auto_blocks = (uint32_t)bpc * (uint32_t)ctx->cu_count;
+    if (serve) {
+        // A server is idle most of the time, and every idle wave polls the
+        // launch-wide words.  kServeWorkersPerCU workers per CU served the
+        // mixed request stream (1,000 requests/s) faster than the occupancy
+        // grid (856 vs 554 graphs/s; small requests p50 0.26 vs 4.3 ms, p99
+        // 5.8 vs 37 ms), and the CUs left free run other kernels -- with the
+        // occupancy grid a vc_solve issued beside a server waited for it.
+        auto_blocks = std::min<uint32_t>(
+            auto_blocks, std::max<uint32_t>(1u, kServeWorkersPerCU * (uint32_t)ctx->cu_count / pl.wpb));
+    }
+    pl.num_blocks = p->num_blocks ? p->num_blocks : auto_blocks;
 
     pl.depth = std::max<uint32_t>(depth, 1u);
 
@@ -2130,16 +2169,17 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     bool want_covers = false;
     if (covers)
         for (int i = 0; i < set->count; ++i) want_covers |= covers[i] != nullptr;
-    // pinned readback buffers
+    // pinned readback buffers, mapped: publish_kernel writes them
+    constexpr unsigned kPinned = hipHostMallocCoherent | hipHostMallocMapped;
     bool ok = true;
     if (!set->h_state)
-        ok = hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_state), sizeof(GState) * set->count)) &&
-             hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_glob), sizeof(Global))) &&
-             hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_cov), sizeof(uint32_t) * set->cover_words));
+        ok = hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_state), sizeof(GState) * set->count, kPinned)) &&
+             hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_glob), sizeof(Global), kPinned)) &&
+             hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_cov), sizeof(uint32_t) * set->cover_words, kPinned));
     if (ok && set->ws_cap < pl.workers) {
         ctx_release(ctx, set->h_ws, true);
         set->h_ws = nullptr;
-        ok = hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_ws), sizeof(vcg::WorkerStats) * pl.workers));
+        ok = hip_ok(hipHostMalloc(reinterpret_cast<void **>(&set->h_ws), sizeof(vcg::WorkerStats) * pl.workers, kPinned));
         set->ws_cap = ok ? pl.workers : 0;
     }
     if (!ok) return VC_ENOMEM;
@@ -2151,15 +2191,24 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         if (timed && !hip_ok(hipEventRecord(ctx->ev0, ctx->stream))) return false;
         if (!launch_solve(pl, ctx->stream, a)) return false;
         if (timed && !hip_ok(hipEventRecord(ctx->ev1, ctx->stream))) return false;
-        bool r = hip_ok(hipMemcpyAsync(set->h_state, set->d_state, sizeof(GState) * set->count, hipMemcpyDeviceToHost,
-                                       ctx->stream)) &&
-                 hip_ok(hipMemcpyAsync(set->h_glob, a.glob, sizeof(Global), hipMemcpyDeviceToHost, ctx->stream)) &&
-                 hip_ok(hipMemcpyAsync(set->h_ws, a.wstats, sizeof(vcg::WorkerStats) * pl.workers,
-                                       hipMemcpyDeviceToHost, ctx->stream));
-        if (r && want_covers)
-            r = hip_ok(hipMemcpyAsync(set->h_cov, set->d_covers, sizeof(uint32_t) * set->cover_words,
-                                      hipMemcpyDeviceToHost, ctx->stream));
-        return r;
+        vcg::Publish pub;
+        std::memset(&pub, 0, sizeof(pub));
+        pub.src[0] = reinterpret_cast<const uint32_t *>(set->d_state);
+        pub.dst[0] = reinterpret_cast<uint32_t *>(set->h_state);
+        pub.words[0] = (uint32_t)(sizeof(GState) / 4u * (size_t)set->count);
+        pub.src[1] = reinterpret_cast<const uint32_t *>(a.glob);
+        pub.dst[1] = reinterpret_cast<uint32_t *>(set->h_glob);
+        pub.words[1] = (uint32_t)(sizeof(Global) / 4u);
+        pub.src[2] = reinterpret_cast<const uint32_t *>(a.wstats);
+        pub.dst[2] = reinterpret_cast<uint32_t *>(set->h_ws);
+        pub.words[2] = (uint32_t)(sizeof(vcg::WorkerStats) / 4u * (size_t)pl.workers);
+        if (want_covers) {
+            pub.src[3] = set->d_covers;
+            pub.dst[3] = set->h_cov;
+            pub.words[3] = (uint32_t)set->cover_words;
+        }
+        hipLaunchKernelGGL(vcg::publish_kernel, dim3(64), dim3(256), 0, ctx->stream, pub);
+        return hip_ok(hipGetLastError());
     };
 
     if (params->flags & VC_FLAG_GRAPH) {
@@ -2453,6 +2502,7 @@ struct vc_server {
     // pinned host memory shared with the kernel
     vcg::HostCtrl *host = nullptr;
     vcg::Completion *comp = nullptr;
+    vcg::WorkerStats *h_ws = nullptr;  // per-worker statistics, written at kernel exit
     GDesc *h_desc = nullptr;
     uint32_t *h_ring = nullptr;
     uint32_t *h_arena = nullptr;
@@ -2624,6 +2674,7 @@ void server_free(vc_server *s)
     ctx_release(ctx, s->d_arena, false);
     ctx_release(ctx, s->host, true);
     ctx_release(ctx, s->comp, true);
+    ctx_release(ctx, s->h_ws, true);
     ctx_release(ctx, s->h_desc, true);
     ctx_release(ctx, s->h_ring, true);
     ctx_release(ctx, s->h_arena, true);
@@ -2683,6 +2734,7 @@ vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **
               hip_ok(hipMalloc(&s->scratch, s->pl.total)) && hip_ok(hipMalloc(&s->d_desc, sizeof(GDesc) * slots)) &&
               hip_ok(hipMalloc(&s->d_state, sizeof(GState) * slots)) && hip_ok(hipMalloc(&s->d_arena, arena_bytes)) &&
               pinned_alloc(&s->host, sizeof(vcg::HostCtrl)) && pinned_alloc(&s->comp, sizeof(vcg::Completion) * slots) &&
+              pinned_alloc(&s->h_ws, sizeof(vcg::WorkerStats) * s->pl.workers) &&
               pinned_alloc(&s->h_desc, sizeof(GDesc) * slots) && pinned_alloc(&s->h_ring, sizeof(uint32_t) * s->ring_size) &&
               pinned_alloc(&s->h_arena, arena_bytes);
     if (!ok) {
@@ -2712,6 +2764,7 @@ vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **
     a.host_arena = s->h_arena;
     a.host = s->host;
     a.comp = s->comp;
+    a.wstats = s->h_ws;
     a.poll_ticks = std::max<uint64_t>(1, (uint64_t)ctx->wall_khz * 2u / 1000u);  // about 2 microseconds
     // A server is idle most of the time: its idle waves poll about every
     // 220 us, and with thousands of them polling at random phases a new
@@ -2806,24 +2859,14 @@ vc_status vc_server_stop(vc_server *s, vc_launch_info *info)
     }
     if (s->th.joinable()) s->th.join();
     vc_status st = hip_ok(hipEventSynchronize(s->ev1)) ? VC_OK : VC_EDEVICE;
-    std::vector<vcg::WorkerStats> wn(s->pl.workers);
-    Global hg;
-    std::memset(&hg, 0, sizeof(hg));
-    // statistics through this server's stream, waited for by an event (see
-    // server_loop)
-    hipEvent_t done = nullptr;
-    bool got = hip_ok(hipEventCreateWithFlags(&done, hipEventDisableTiming)) &&
-               hip_ok(hipMemcpyAsync(wn.data(), s->a.wstats, sizeof(vcg::WorkerStats) * s->pl.workers,
-                                     hipMemcpyDeviceToHost, s->kstream)) &&
-               hip_ok(hipMemcpyAsync(&hg, s->a.glob, sizeof(Global), hipMemcpyDeviceToHost, s->kstream)) &&
-               hip_ok(hipEventRecord(done, s->kstream)) && hip_ok(hipEventSynchronize(done));
-    if (done) (void)hipEventDestroy(done);
-    if (!got) st = VC_EDEVICE;
-    if (hg.error) st = VC_EDEVICE;
+    // the kernel has ended: its statistics and error word are in pinned host
+    // memory (a copy here could wait for another server's persistent kernel)
+    if (__atomic_load_n(&s->host->error, __ATOMIC_ACQUIRE) != 0u) st = VC_EDEVICE;
     if (info) {
         std::memset(info, 0, sizeof(*info));
         uint64_t idle = 0;
-        for (const auto &x : wn) {
+        for (uint32_t i = 0; i < s->pl.workers; ++i) {
+            const vcg::WorkerStats &x = s->h_ws[i];
             info->tree_nodes += x.nodes;
             idle += x.idle_ticks;
             info->donations += x.donations;

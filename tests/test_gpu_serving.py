@@ -126,9 +126,14 @@ def test_server_admits_by_stack_footprint(ctx):
 def test_hbm_server_beside_a_full_small_server(ctx):
     """Two persistent servers share the GPU: one whose node storage is in
     HBM (max_vertices 20000 does not fit LDS) and a small-graph server with
-    the occupancy-derived grid.  The larger-footprint server starts first
-    (a persistent kernel launched second only gets the CUs left over).  Both
-    answer exactly while both run."""
+    the full occupancy grid (asked for explicitly: the default grid is
+    smaller).  The larger-footprint server starts first (a persistent kernel
+    launched second only gets the CUs left over).  Both answer exactly while
+    both run, and stopping one does not wait for the other's kernel."""
+    probe = vcgpu.Server(ctx, max_vertices=512, slots=64, time_limit_ms=60000)
+    pi = probe.stop()
+    full_blocks = pi.blocks_per_cu * pi.cu_count
+    assert pi.num_blocks < full_blocks  # the default grid leaves CUs free
     big = []
     for s in range(2):
         n, e = 3000, graphgen.barabasi_albert(3000, 3, 900 + s)
@@ -140,7 +145,7 @@ def test_hbm_server_beside_a_full_small_server(ctx):
     depth = max(min(g.n, U + 1) + 1 for g, _, U, _ in big)  # PVC at k = opt = U needs U + 2
     large = vcgpu.Server(ctx, max_vertices=20000, slots=8, block_dim=64, num_blocks=8, stack_depth=depth,
                          time_limit_ms=60000)
-    small = vcgpu.Server(ctx, max_vertices=512, slots=64, time_limit_ms=60000)
+    small = vcgpu.Server(ctx, max_vertices=512, slots=64, num_blocks=full_blocks, time_limit_ms=60000)
     try:
         reqs = _requests()[:16]
         ts = [small.submit(g, job) for g, _, job, _, _ in reqs]
@@ -162,6 +167,31 @@ def test_hbm_server_beside_a_full_small_server(ctx):
             assert ryes.solved and ryes.k_found == 1 and ryes.cover is not None
             assert O.cover_is_valid(og, np.asarray(ryes.cover, np.uint8)) and int(np.sum(ryes.cover)) <= opt
     finally:
+        t_stop = time.monotonic()
         info_l = large.stop()
+        t_stop = time.monotonic() - t_stop
         small.stop()
     assert info_l.use_global_memory == 1
+    assert t_stop < 10.0, t_stop  # the small server's kernel runs on (60 s limit)
+
+
+def test_solve_beside_a_default_server_does_not_wait(ctx):
+    """A batch solve issued while a server with the default grid runs gets
+    CUs of its own and returns at once (a server that filled every CU would
+    hold it until the server's time limit)."""
+    n, e = 120, graphgen.regular3_rewired(120, 3)
+    og = O.from_edges(n, e)
+    opt, _, _ = O.mvc(og)
+    g = vcgpu.csr_from_edges(n, e)
+    U, _ = vcgpu.upper_bound(g, seed=1)
+    gs = ctx.upload(g)
+    srv = vcgpu.Server(ctx, max_vertices=512, slots=64, time_limit_ms=60000)
+    try:
+        t = time.monotonic()
+        (r,), info, st = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), vcgpu.Params(time_limit_ms=30000),
+                                  raise_on_error=False)
+        t = time.monotonic() - t
+    finally:
+        srv.stop()
+    assert r.solved and r.minimum == opt
+    assert t < 10.0, t
