@@ -3,7 +3,7 @@
 //
 // One wave64 is one search worker.  A workgroup holds several independent
 // workers; no workgroup barrier is ever used.  Each worker keeps the degree
-// array of its current tree node (-1 = vertex in the cover) in LDS -- or, for
+// array of its current tree node (negative = vertex in the cover) in LDS -- or, for
 // graphs too large for LDS, in a per-worker HBM scratch area (the reference's
 // `_global_` kernels) -- plus a mark bitset and a compact mark list used by
 // the reduction rules.  Degrees are int16 when every graph of the launch has
