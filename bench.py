@@ -489,6 +489,10 @@ def main():
                        "tree_nodes_per_s": nodes / wall, "kernel_ms": kernel_s * 1000.0,
                        "workers": info.workers, "block_dim": info.block_dim, "num_blocks": info.num_blocks,
                        "blocks_per_cu": info.blocks_per_cu, "status": st,
+                       # resident waves (one per worker; the persistent grid never exceeds
+                       # occupancy) per SIMD, and as a fraction of the 8 wave slots of a SIMD
+                       "waves_per_simd": info.workers / (info.cu_count * 4) if info.cu_count else None,
+                       "wave_occupancy": info.workers / (info.cu_count * 4 * 8) if info.cu_count else None,
                        "idle_fraction": info.idle_fraction, "donations": info.donations, "steals": info.steals,
                        "tuned": tuned, "graph_captured": bool(params.flags & vcgpu.VC_FLAG_GRAPH),
                        "plain_graphs_per_s": plain_solved / plain_wall, "plain_sample": ps,

@@ -278,3 +278,41 @@ def test_upload_rejects_what_the_kernels_cannot_solve(ctx):
     gs = ctx.upload(g)  # the well-formed graph still uploads and solves
     (r,), _, _ = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, 2, 0), P())
     assert r.solved and r.minimum == 2
+
+
+def test_golden_vectors_on_the_gpu(ctx):
+    """Every committed golden vector (seeded graphs whose U, MVC and PVC
+    answers the compiled reference confirmed, tests/golden/oracle_vectors.json)
+    in one batched launch: MVC from RemoveMax's U equals the recorded MVC,
+    and PVC at every recorded k equals [k >= MVC] -- which is the reference's
+    own SequentialParameterized answer whenever k < U (for k >= U that CPU
+    solver passes U to the high-degree rule and can answer "no";
+    src/SequentialParameterized.cpp:45)."""
+    import json
+    import os
+    from golden.make_golden import make_graph
+    here = os.path.dirname(os.path.abspath(__file__))
+    vecs = json.load(open(os.path.join(here, "golden", "oracle_vectors.json")))
+    graphs, jobs, expect = [], [], []
+    for v in vecs:
+        n, e = make_graph(v["kind"], v["spec"])
+        assert graphgen.edge_checksum(e) == v["edge_sha256"], v["key"]
+        g = vcgpu.csr_from_edges(n, e)
+        graphs.append(g)
+        jobs.append(vcgpu.Job(vcgpu.VC_MVC, 0, v["U_max"], 0))
+        expect.append((v, None, None))
+        for k, rec in sorted(v["pvc"].items(), key=lambda kv: int(kv[0])):
+            graphs.append(g)
+            jobs.append(vcgpu.Job(vcgpu.VC_PVC, int(k), v["U_max"], 0))
+            expect.append((v, int(k), rec["kFound"]))
+    gs = ctx.upload(graphs)
+    res, info, st = ctx.solve(gs, jobs, P(time_limit_ms=60000))
+    assert st == vcgpu.VC_OK
+    for (v, k, ref_found), r in zip(expect, res):
+        assert r.solved, v["key"]
+        if k is None:
+            assert r.minimum == v["mvc"], (v["key"], r.minimum, v["mvc"])
+        else:
+            assert r.k_found == int(k >= v["mvc"]), (v["key"], k, r.k_found)
+            if k < v["U_max"]:
+                assert r.k_found == ref_found, (v["key"], k, r.k_found, ref_found)
