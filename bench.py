@@ -66,6 +66,11 @@ def _code_only(text: str) -> str:
     return " ".join("".join(out).split())
 
 
+def pct(a, q):
+    """q-th percentile of a sequence, None when it is empty."""
+    return float(np.percentile(a, q)) if len(a) else None
+
+
 def kernel_source_sha() -> str:
     """Hash of the kernel sources (code only) a committed profile was measured on."""
     import hashlib
@@ -339,7 +344,6 @@ def served(ctx, args, out):
         exact[cls] += int(ok)
     wall = (last - start) / 1e9
     both = np.asarray(lat["small"] + lat["large"])
-    pct = lambda a, q: float(np.percentile(a, q)) if len(a) else None  # noqa: E731
     solved = exact["small"] + exact["large"]
     out.update({
         "value": solved / wall, "unit": "graphs/s", "ms_per_step": wall * 1000.0,
@@ -471,9 +475,12 @@ def main():
         ones = [ctx.upload(graphs[i]) for i in range(ps)]
         plain_params = vcgpu.Params(version=vcgpu.VC_HYBRID, time_limit_ms=args.time_limit_ms)
         t0 = time.perf_counter()
-        plain_solved = 0
+        plain_solved, plain_lat = 0, []
         for i in range(ps):
+            t1 = time.perf_counter()
             (r,), _, _ = ctx.solve(ones[i], jobs[i], plain_params, want_covers=False, raise_on_error=False)
+            if r.solved:
+                plain_lat.append((time.perf_counter() - t1) * 1000.0)
             plain_solved += int(r.solved)
         plain_wall = max(time.perf_counter() - t0, 1e-9)
         for one in ones:
@@ -500,7 +507,13 @@ def main():
                        "upload_ms": upload_s * 1000.0,
                        "graphs_per_s_with_upload": solved / (wall + upload_s),
                        "batched_over_plain": ((solved / wall) / (plain_solved / plain_wall)
-                                              if ps and plain_solved else None)},
+                                              if ps and plain_solved else None),
+                       # latency of the solved graphs: batched = when a graph's search ended inside
+                       # the one launch (all 4096 submitted together, measured from its first
+                       # worker's start); plain = the wall time of one graph's own launch
+                       "batched_p50_ms": pct([r.done_us / 1000.0 for r in res if r.solved], 50),
+                       "batched_p99_ms": pct([r.done_us / 1000.0 for r in res if r.solved], 99),
+                       "plain_p50_ms": pct(plain_lat, 50), "plain_p99_ms": pct(plain_lat, 99)},
             "roofline": {"bound": "hbm", "achieved": alg_bytes / kernel_s / 1e9, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": alg_bytes / kernel_s / 1e9 / HBM_PEAK_GBS,
                          "traffic": traffic, "traffic_source": traffic_src,

@@ -100,7 +100,9 @@ typedef struct {
                                 untouched (the caller holds the U-cover) */
     uint64_t tree_nodes;     /* node-loop iterations spent on this graph */
     int32_t status;          /* vc_status for this graph */
-    int32_t reserved;
+    uint32_t done_us;        /* vc_solve: when this graph's search ended, in
+                                microseconds after the launch's first worker
+                                started (0 from a server) */
 } vc_result;
 
 /* Launch facts of the last vc_solve (the reference prints these,

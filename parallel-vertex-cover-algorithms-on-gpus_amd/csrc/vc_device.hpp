@@ -42,7 +42,9 @@ struct GState {
     int32_t cover_size;   // size of the cover held in the bitmap, -1 = none
     uint32_t done;        // outstanding reached 0
     uint32_t budget_hit;  // stop was set by the node budget
-    uint32_t pad[6];
+    uint32_t pad0;
+    uint64_t done_tick;   // device wall clock when outstanding reached 0
+    uint32_t pad[2];
 };
 static_assert(sizeof(GState) == 64, "GState is one 64-byte line");
 

@@ -1063,7 +1063,10 @@ __device__ void finish_unit(const SolveArgs &a, uint32_t g, const GDesc &gd)
         uint32_t old = __hip_atomic_fetch_add(&gs->outstanding, 0xFFFFFFFFu, __ATOMIC_ACQ_REL,
                                               __HIP_MEMORY_SCOPE_AGENT);
         last = old == 1u;
-        if (last) st_rlx(&gs->done, 1u);
+        if (last) {
+            st_rlx(&gs->done_tick, (uint64_t)wall_clock64());
+            st_rlx(&gs->done, 1u);
+        }
     }
     if constexpr (!SERVE) {
         return;
@@ -2293,6 +2296,9 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
             r.cover_from_gpu = s.found != 0u;
         }
         r.status = hg.error ? VC_EDEVICE : (r.solved ? VC_OK : (hg.cancel && !hg.timed_out ? VC_ECANCELLED : VC_ETIMEOUT));
+        if (s.done && s.done_tick >= hg.t0 && hg.t0 != 0u && ctx->wall_khz > 0)
+            r.done_us = (uint32_t)std::min<uint64_t>((s.done_tick - hg.t0) * 1000u / (uint64_t)ctx->wall_khz,
+                                                     0xFFFFFFFFull);
         if (covers && covers[i] && r.cover_from_gpu) {
             const uint32_t *bm = &bitmaps[set->h_desc[i].off_cover];
             for (uint32_t v = 0; v < set->h_desc[i].n; ++v) covers[i][v] = (bm[v >> 5] >> (v & 31u)) & 1u;

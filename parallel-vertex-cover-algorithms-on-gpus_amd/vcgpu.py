@@ -53,7 +53,7 @@ class _Params(C.Structure):
 class _Result(C.Structure):
     _fields_ = [("minimum", C.c_uint32), ("k_found", C.c_uint32), ("solved", C.c_uint32),
                 ("cover_from_gpu", C.c_uint32), ("tree_nodes", C.c_uint64), ("status", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("done_us", C.c_uint32)]
 
 
 class _Info(C.Structure):
@@ -228,6 +228,7 @@ class Result:
     tree_nodes: int
     status: int
     cover: np.ndarray | None
+    done_us: int = 0  # vc_solve: search end, microseconds after the launch's first worker started
 
 
 @dataclass
@@ -372,7 +373,8 @@ class Context:
         for i in range(G):
             r = res[i]
             cover = bufs[i][: gset.graphs[i].n].astype(bool) if (want_covers and r.cover_from_gpu) else None
-            out.append(Result(int(r.minimum), int(r.k_found), bool(r.solved), int(r.tree_nodes), int(r.status), cover))
+            out.append(Result(int(r.minimum), int(r.k_found), bool(r.solved), int(r.tree_nodes), int(r.status), cover,
+                              int(r.done_us)))
         li = LaunchInfo(*[getattr(info, f) for f, _ in _Info._fields_])
         return out, li, st
 

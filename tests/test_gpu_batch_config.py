@@ -24,6 +24,9 @@ def test_batched_configuration_every_answer_checks(ctx):
     assert st in (vcgpu.VC_OK, vcgpu.VC_ETIMEOUT)
     solved = [i for i, r in enumerate(res) if r.solved]
     assert len(solved) > 2000, len(solved)
+    # each graph's end time lies inside the launch (first worker start to kernel end)
+    done = [res[i].done_us for i in solved]
+    assert max(done) > 0 and max(done) <= info.kernel_ms * 1000.0 * 1.02 + 1000.0, (max(done), info.kernel_ms)
     for i, ((n, e), r, (U, ucov)) in enumerate(zip(graphs, res, ubs)):
         e = np.asarray(e, np.int64).reshape(-1, 2)
         if r.solved:
