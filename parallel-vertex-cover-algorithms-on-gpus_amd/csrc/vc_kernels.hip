@@ -471,6 +471,27 @@ __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
 {
     L = uni(L);
     w.sync();
+    if (L == 1u) {
+        // one listed vertex (most leaf passes): its id and adjacency bounds
+        // are wave-uniform, so no list walk, prefix scan or owner search
+        const uint32_t u0 = uni(w.list[0]);
+        if (lane_id() == 0) w.remove(u0);
+        w.sync();
+        const uint32_t b0 = uni(w.src[u0]), e0 = uni(w.src[u0 + 1]);
+#pragma unroll 1
+        for (uint32_t x0 = b0; x0 < e0; x0 += kWave) {
+            const uint32_t x = x0 + lane_id();
+            uint32_t leaf = kNone;
+            if (x < e0) {
+                const uint32_t t = w.dst[x];
+                if (w.drop(t, u0) == 2) leaf = t;
+            }
+            nc = add_leaf_cands(w, nc, leaf);
+        }
+        if (lane_id() == 0) w.mark[u0 >> 5] = 0u;
+        w.sync();
+        return;
+    }
     for (uint32_t i = lane_id(); i < L; i += kWave) w.remove(w.list[i]);
     w.sync();
     const int lane = (int)lane_id();

@@ -316,3 +316,32 @@ def test_golden_vectors_on_the_gpu(ctx):
             assert r.k_found == int(k >= v["mvc"]), (v["key"], k, r.k_found)
             if k < v["U_max"]:
                 assert r.k_found == ref_found, (v["key"], k, r.k_found, ref_found)
+
+
+@pytest.mark.parametrize("shape", ["k2000", "k300", "k150_150", "k300_wide"])
+def test_dense_closed_forms(ctx, shape):
+    """Dense graphs whose vertices, once in the cover, keep being decremented
+    by every neighbour that follows them (up to n - 2 times in K_n): the
+    degree stays negative, so MVC and cover are exact.  K_n has MVC n - 1,
+    K_{a,a} has MVC a (Konig).  With U = n the device search must find the
+    cover itself (RemoveMax's U is already optimal on these graphs); K_2000
+    keeps RemoveMax's U and exercises passes capped at the list capacity."""
+    flags = 0
+    if shape.startswith("k150"):
+        a = 150
+        x, y = np.meshgrid(np.arange(a), np.arange(a, 2 * a), indexing="ij")
+        n, e, want = 2 * a, np.stack([x.ravel(), y.ravel()], 1), a
+    else:
+        n = 2000 if shape == "k2000" else 300
+        iu, ju = np.triu_indices(n, 1)
+        e, want = np.stack([iu, ju], 1), n - 1
+        flags = vcgpu.VC_FLAG_WIDE if shape.endswith("wide") else 0
+    g = vcgpu.csr_from_edges(n, e)
+    U = vcgpu.upper_bound(g, seed=1)[0] if shape == "k2000" else n
+    gs = ctx.upload(g)
+    (r,), info, st = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), P(flags=flags))
+    assert r.solved and r.minimum == want, (shape, r.minimum, want)
+    if shape != "k2000":
+        assert r.cover is not None
+        cover = np.asarray(r.cover, bool)
+        assert int(cover.sum()) == want and bool(np.all(cover[e[:, 0]] | cover[e[:, 1]]))
