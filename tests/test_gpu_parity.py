@@ -205,6 +205,23 @@ def test_reference_known_answers(ctx, case):
     assert O.cover_is_valid(og, np.asarray(cover, np.uint8)) and int(np.sum(cover)) == mvc
 
 
+@pytest.mark.parametrize("case", [k for k in _known() if k[3]["cpu_confirmed"]], ids=lambda k: k[0])
+def test_reference_known_answers_pvc(ctx, case):
+    """The same table as a decision problem: no cover of size MVC - 1, one of
+    size MVC (valid, from the device), in one launch."""
+    name, n, e, meta = case
+    opt = meta["mvc"]
+    g = vcgpu.csr_from_edges(n, e)
+    og = O.from_edges(n, e)
+    U, _ = vcgpu.upper_bound(g, seed=1)
+    gs = ctx.upload([g, g])
+    (no, yes), info, st = ctx.solve(gs, [vcgpu.Job(vcgpu.VC_PVC, opt - 1, U, 0), vcgpu.Job(vcgpu.VC_PVC, opt, U, 0)],
+                                    P(time_limit_ms=60000))
+    assert no.solved and no.k_found == 0, name
+    assert yes.solved and yes.k_found == 1 and yes.cover is not None, name
+    assert O.cover_is_valid(og, np.asarray(yes.cover, np.uint8)) and int(np.sum(yes.cover)) <= opt, name
+
+
 @pytest.mark.parametrize("case", [k for k in _known() if not k[3]["cpu_confirmed"]], ids=lambda k: k[0])
 def test_reference_known_answers_hard(ctx, case):
     """Graphs of the same table the CPU oracle cannot finish in minutes: with
