@@ -158,6 +158,9 @@ struct SolveArgs {
     // thousands of idle waves polling often load the memory channels that
     // the busy waves (and other kernels) use
     uint32_t idle_cap;
+    // 1 = every graph of the launch has 2m < 2^16 and node storage is in
+    // LDS: a worker keeps its current graph's srcPtr there as uint16
+    uint32_t off16;
 };
 
 }  // namespace vcg
