@@ -1378,9 +1378,9 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 // pop (popStack, src/stack.cuh:10-20), then build the stored
                 // right child
                 const uint32_t ts = (uint32_t)top % a.depth;
+                const uint2 m = stk_meta[ts];  // in flight during the copy
                 copy_state<NodeT>(w.base, stk + (size_t)ts * slot_bytes, a.npad, w.n);
                 w.sync();
-                uint2 m = stk_meta[ts];
                 d = uni(m.x);
                 uint32_t v = uni(m.y);
                 --top;
@@ -1438,7 +1438,11 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         }
 
         GState *gs = &a.gstate[g];
-        if (uni(ld_rlx(&gs->stop))) {
+        // the stop flag and the incumbent in flight together (one round
+        // trip to L2 per node instead of two)
+        const uint32_t stop_now = ld_rlx(&gs->stop);
+        const uint32_t best_now = (uint32_t)ld_rlx(&gs->best);
+        if (uni(stop_now)) {
             // graph finished by someone else (PVC found, budget): drop local work
             top = -1;
             bot = 0;
@@ -1459,7 +1463,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         }
 
         const bool mvc = gd.instance == 0;
-        uint32_t T = mvc ? (uint32_t)uni((uint32_t)ld_rlx(&gs->best)) : gd.k + 1u;
+        uint32_t T = mvc ? uni(best_now) : gd.k + 1u;
         Summary s;
         d = reduce_node(w, d, T, &s, nc, full);
 
