@@ -849,6 +849,7 @@ __device__ uint32_t take_neighbours(const N &w, uint32_t v, uint32_t &nc)
         if (L == 0) break;
         apply_list(w, L, nc);
         total += L;
+        if (L < w.lcap) break;  // no entry was dropped: every live neighbour is done
     }
     return total;
 }
