@@ -335,21 +335,23 @@ def test_golden_vectors_on_the_gpu(ctx):
                 assert r.k_found == ref_found, (v["key"], k, r.k_found, ref_found)
 
 
-@pytest.mark.parametrize("shape", ["k2000", "k300", "k150_150", "k300_wide"])
+@pytest.mark.parametrize("shape", ["k2000", "k300", "k150_150", "k300_wide", "k256", "k257"])
 def test_dense_closed_forms(ctx, shape):
     """Dense graphs whose vertices, once in the cover, keep being decremented
     by every neighbour that follows them (up to n - 2 times in K_n): the
     degree stays negative, so MVC and cover are exact.  K_n has MVC n - 1,
     K_{a,a} has MVC a (Konig).  With U = n the device search must find the
     cover itself (RemoveMax's U is already optimal on these graphs); K_2000
-    keeps RemoveMax's U and exercises passes capped at the list capacity."""
+    keeps RemoveMax's U and exercises passes capped at the list capacity.
+    K_256 (2m = 65,280) runs with 16-bit adjacency offsets in LDS, K_257
+    (2m = 65,792) just past that limit with the offsets read from HBM."""
     flags = 0
     if shape.startswith("k150"):
         a = 150
         x, y = np.meshgrid(np.arange(a), np.arange(a, 2 * a), indexing="ij")
         n, e, want = 2 * a, np.stack([x.ravel(), y.ravel()], 1), a
     else:
-        n = 2000 if shape == "k2000" else 300
+        n = {"k2000": 2000, "k256": 256, "k257": 257}.get(shape, 300)
         iu, ju = np.triu_indices(n, 1)
         e, want = np.stack([iu, ju], 1), n - 1
         flags = vcgpu.VC_FLAG_WIDE if shape.endswith("wide") else 0
