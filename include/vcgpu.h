@@ -67,8 +67,12 @@ typedef struct {
  * fields that shape the device solve.  0 / -1 mean "pick automatically". */
 typedef struct {
     uint32_t version;           /* vc_version (-v) */
-    uint32_t block_dim;         /* threads per workgroup (-b); 0 = auto */
-    uint32_t num_blocks;        /* workgroups (-n); 0 = auto (occupancy x CUs) */
+    uint32_t block_dim;         /* threads per workgroup (-b): a multiple of 64 up to 1024,
+                                   one search worker per 64-thread wave; 0 = auto.  When an
+                                   explicit workgroup's workers do not fit in LDS their node
+                                   state goes to HBM (as with -g 1) */
+    uint32_t num_blocks;        /* workgroups (-n); 0 = auto (occupancy x CUs).  At most
+                                   65535 workers (num_blocks x block_dim / 64) per launch */
     int32_t use_global_memory;  /* -g: -1 auto, 0 degree arrays in LDS, 1 in HBM */
     uint32_t list_size;         /* global worklist slots (-q, power of 2); 0 = auto */
     float list_threshold;       /* donate while queued < threshold*size (-t); <=0 = auto */
@@ -191,7 +195,7 @@ typedef struct {
                                (at most the occupancy): idle waves poll, so a
                                small grid serves a request stream faster and
                                leaves CUs to other kernels */
-    uint32_t block_dim;     /* threads per workgroup (multiple of 64, <= 256); 0 = auto */
+    uint32_t block_dim;     /* threads per workgroup (multiple of 64, <= 1024); 0 = auto */
     uint32_t stack_depth;   /* stack entries per worker; 0 = max_vertices + 1 */
     uint32_t time_limit_ms; /* lifetime limit of the server kernel; 0 = one hour */
 } vc_server_config;

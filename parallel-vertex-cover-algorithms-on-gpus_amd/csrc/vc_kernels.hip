@@ -27,6 +27,7 @@
 // on blockDim, src/helperFunctions.cuh:453-478).
 #include <hip/hip_runtime.h>
 
+#include "vc_bound.hpp"
 #include "vc_device.hpp"
 
 namespace vcg {
@@ -1038,20 +1039,30 @@ __device__ void record_cover(const SolveArgs &a, const N &w, const GDesc &gd, ui
         if (lane_id() == 0) old = atomicMin(&gs->best, (int)d);
         old = (int)uni((uint32_t)old);
         if ((int)d < old) {
+            // A wait cut short by the deadline leaves the lock to its holder:
+            // no cover is written (the launch ends unfinished anyway) and the
+            // lock is not released by a wave that does not own it.
+            uint32_t owned = 0;
             if (lane_id() == 0) {
-                while (atomicCAS(&gs->cover_lock, 0u, 1u) != 0u) {
+                for (;;) {
+                    if (atomicCAS(&gs->cover_lock, 0u, 1u) == 0u) {
+                        owned = 1;
+                        break;
+                    }
                     backoff(2);
                     if (past_deadline(a)) break;
                 }
             }
-            fence_acq();
-            int cur = (int)uni((uint32_t)ld_rlx(&gs->best));
-            if (cur == (int)d) {
-                write_cover(a, w, gd);
-                if (lane_id() == 0) st_rlx(&gs->cover_size, (int32_t)d);
+            if (uni(owned)) {
+                fence_acq();
+                int cur = (int)uni((uint32_t)ld_rlx(&gs->best));
+                if (cur == (int)d) {
+                    write_cover(a, w, gd);
+                    if (lane_id() == 0) st_rlx(&gs->cover_size, (int32_t)d);
+                }
+                fence_rel();
+                if (lane_id() == 0) st_rel(&gs->cover_lock, 0u);
             }
-            fence_rel();
-            if (lane_id() == 0) st_rel(&gs->cover_lock, 0u);
         }
     } else {
         // PVC: atomicOr(kFound, 1) (src/GlobalWorkListParameterized.cuh:148-156);
@@ -1275,8 +1286,6 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
     }
 }
 
-__device__ __forceinline__ long long sq64(int x) { return (long long)x * (long long)x; }
-
 // A graph's descriptor with every field made wave-uniform (scalar
 // registers): loop bounds such as n then compile to scalar compares instead
 // of per-lane compares with exec-mask bookkeeping.
@@ -1468,14 +1477,10 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         Summary s;
         d = reduce_node(w, d, T, &s, nc, full);
 
-        bool prune;
-        if (mvc) {
-            int best = (int)uni((uint32_t)ld_rlx(&gs->best));
-            prune = (int)d >= best || (long long)s.edges >= sq64(best - (int)d - 1) + 1;
-        } else {
-            prune = d > gd.k || (long long)s.edges >= sq64((int)gd.k - (int)d) + 1;
-        }
-        if (prune) {
+        // the live incumbent (another worker may have lowered it while the
+        // rules ran with T); the bound uses both (bound_prunes)
+        const uint32_t B = mvc ? uni((uint32_t)ld_rlx(&gs->best)) : T;
+        if (bound_prunes(s.edges, d, T, B)) {
             have = false;
             descend = 0;
             continue;
@@ -1615,7 +1620,7 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
 }
 
 template <typename DegT, bool HBM, bool SERVE, bool OFF16 = false>
-__global__ __launch_bounds__(256) void solve_kernel(SolveArgs a)
+__global__ __launch_bounds__(1024) void solve_kernel(SolveArgs a)
 {
     solve_body<DegT, HBM, SERVE, OFF16>(a);
 }
@@ -1626,7 +1631,7 @@ template __global__ void solve_kernel<int16_t, false, false>(SolveArgs);
 // per SIMD); asking for 6 waves per EU gets 80 with one register spilled.
 // Measured per batched step: 3.67 s against 4.15 s at 5 waves and 3.88 s
 // for the HBM-offset kernel at 7 waves.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void solve_kernel_off16(SolveArgs a)
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void solve_kernel_off16(SolveArgs a)
 {
     solve_body<int16_t, false, false, true>(a);
 }
@@ -1845,6 +1850,8 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = fals
 // Launch shape shared by the batch solve and the server: node storage,
 // workgroup, grid from the occupancy query, worklist, and the memory plan.
 constexpr uint32_t kServeWorkersPerCU = 2;
+constexpr uint32_t kMaxBlockDim = 1024;   // the solve kernels' launch bound
+constexpr uint32_t kMaxWorkers = 0xFFFFu; // 16-bit worker counts in the termination word
 
 vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_params *p, Plan &pl, bool serve,
                       bool off16_ok = false)
@@ -1852,18 +1859,29 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
     const uint32_t version = p->version;
     if (version != VC_HYBRID && version != VC_STACK_ONLY) return VC_EINVAL;
     if (p->use_global_memory < -1 || p->use_global_memory > 1) return VC_EINVAL;
+    // -b: threads per workgroup, 64..1024 as the reference accepts
+    // (src/auxFunctions.cpp:409-444); a worker is one wave, so a workgroup
+    // holds block_dim / 64 independent workers
     uint32_t block_dim = p->block_dim ? p->block_dim : 256u;
-    if (block_dim % vcg::kWave != 0 || block_dim > 256u) return VC_EINVAL;
+    if (block_dim % vcg::kWave != 0 || block_dim > kMaxBlockDim) return VC_EINVAL;
     if (version == VC_STACK_ONLY && p->start_depth > 20u) return VC_EINVAL;
     const uint32_t nmax = std::max<uint32_t>(nmax_in, 1u);
     const size_t lds_limit = ctx->lds_per_cu ? ctx->lds_per_cu : 160u * 1024u;
 
-    // Node storage in LDS when at least one worker fits per CU, otherwise in
-    // HBM (-g 1 forces HBM, -g 0 forbids it); int16 degrees while n < 32768
-    // (VC_FLAG_WIDE forces int32).
+    // Node storage in LDS when the workgroup's workers fit in one CU's LDS
+    // (an automatic workgroup shrinks until they do), otherwise in HBM, as
+    // the reference falls back to its _global_ kernels (-g 1 forces HBM,
+    // -g 0 forbids it); int16 degrees while n < 32768 (VC_FLAG_WIDE forces
+    // int32).
     const bool wide = nmax >= 32768u || (p->flags & VC_FLAG_WIDE) != 0u;
     size_storage(pl, nmax, wide ? 4u : 2u);
-    bool hbm = wide || p->use_global_memory == 1 || pl.worker_bytes > lds_limit;
+    uint32_t wpb = block_dim / vcg::kWave;
+    bool hbm = wide || p->use_global_memory == 1;
+    if (!hbm) {
+        if (!p->block_dim)
+            while (wpb > 1 && (size_t)wpb * pl.worker_bytes > lds_limit) wpb /= 2;
+        hbm = (size_t)wpb * pl.worker_bytes > lds_limit;
+    }
     if (hbm && p->use_global_memory == 0) return VC_ENOTSUP;
     pl.variant = wide ? kHbm32 : hbm ? kHbm16 : kLds16;
     // LDS copy of the current graph's offsets (batch launches whose graphs
@@ -1872,15 +1890,11 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
     if (!serve && off16_ok && pl.variant == kLds16) {
         Plan t = pl;
         size_storage(t, nmax, 2u, true);
-        if (t.worker_bytes <= lds_limit) {
+        if ((size_t)wpb * t.worker_bytes <= lds_limit) {
             pl = t;
             pl.off16 = 1u;
         }
     }
-    // a worker is one wave: shrink the workgroup when LDS would not hold it
-    uint32_t wpb = block_dim / vcg::kWave;
-    if (!hbm && !p->block_dim)
-        while (wpb > 1 && (size_t)wpb * pl.worker_bytes > lds_limit) wpb /= 2;
     pl.wpb = wpb;
     pl.block_dim = wpb * vcg::kWave;
     pl.smem = hbm ? 0 : (size_t)pl.wpb * pl.worker_bytes;
@@ -1904,6 +1918,9 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
             auto_blocks, std::max<uint32_t>(1u, kServeWorkersPerCU * (uint32_t)ctx->cu_count / pl.wpb));
     }
     pl.num_blocks = p->num_blocks ? p->num_blocks : auto_blocks;
+    // the termination word counts waiting and registered workers in 16-bit
+    // fields (vcg::Global::counter)
+    if ((uint64_t)pl.num_blocks * pl.wpb > kMaxWorkers) return VC_EINVAL;
 
     pl.depth = std::max<uint32_t>(depth, 1u);
 

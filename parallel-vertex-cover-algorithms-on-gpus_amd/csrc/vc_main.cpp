@@ -181,6 +181,12 @@ int main(int argc, char **argv)
         vc_csr_free(&owned);
         return 2;
     }
+    if (c.blockDim && (c.blockDim % 64u != 0u || c.blockDim > 1024u)) {
+        // one search worker per 64-lane wave, at most 16 per workgroup
+        std::fprintf(stderr, "-b must be a multiple of 64 from 64 to 1024\n");
+        vc_csr_free(&owned);
+        return 2;
+    }
 
     vc_ctx *ctx = nullptr;
     if ((st = vc_ctx_create(0, &ctx)) != VC_OK) {

@@ -54,7 +54,12 @@ KNOWN = {
 KNOWN_HARD = {
     "cop_hat300-3.txt": 264,
     "cop_hat500-2.txt": 464,
+    "cop_hat500-3.txt": 450,
+    "cop_hat700-2.txt": 656,
+    "cop_hat1000-2.txt": 954,
     "Power-Grid.txt": 2203,
+    "vc-exact_009.txt": 21348,
+    "vc-exact_023.txt": 16013,
     "formated_wikipedia_link_lo.txt": 1536,
 }
 # Files whose vertices are numbered from 0 (the reference's loader,

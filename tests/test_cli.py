@@ -108,3 +108,34 @@ def test_cli_solves_a_zero_indexed_reference_file(tmp_path):
     r = run_vc(["-f", path.name, "-z"], tmp_path)
     assert r.returncode == 0, r.stderr
     assert int(re.search(r"Size of minimum vertex cover: (\d+)", r.stdout).group(1)) == meta["mvc"] == 1077
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block_dim", [512, 1024])
+def test_cli_block_dims_of_the_reference_sweep(tmp_path, block_dim):
+    """-b 512 and -b 1024, the top of the reference harness's BLOCK_DIM sweep
+    (test/runGraphs.py:7; accepted up to 1024 by src/auxFunctions.cpp:409-444):
+    8 or 16 worker waves per workgroup, the same MVC, PVC answers and CSV row."""
+    path, n, e = _graph(tmp_path, n=70, p=0.08, seed=21)
+    opt, _, _ = O.mvc(O.from_edges(n, e))
+    r = run_vc(["-f", path.name, "-b", str(block_dim), "-o", "b"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert int(re.search(r"Size of minimum vertex cover: (\d+)", r.stdout).group(1)) == opt
+    assert f"NumOfThreadPerBlocks : {block_dim}" in r.stdout
+    for k, word in ((opt - 1, "greater than"), (opt, "less than or equal to")):
+        r = run_vc(["-f", path.name, "-i", "pvc", "-k", str(k), "-b", str(block_dim), "-g", "1"], tmp_path)
+        assert r.returncode == 0, r.stderr
+        assert f"Minimum is {word} K: {k}" in r.stdout
+    rows = list(csv.reader(open(tmp_path / "Results" / "Results.csv")))
+    assert len(rows) == 3 and all(len(row) == 22 for row in rows)
+    assert all(int(row[10]) == block_dim for row in rows)
+
+
+def test_cli_rejects_a_workgroup_above_1024(tmp_path):
+    """Workgroups beyond the kernels' 1024-thread launch bound, or not a whole
+    number of 64-lane waves, are refused with an error line (no device
+    work is started for them)."""
+    path, _, _ = _graph(tmp_path)
+    for b in ("2048", "96"):
+        r = run_vc(["-f", path.name, "-b", b], tmp_path)
+        assert r.returncode == 2 and "-b must be a multiple of 64" in r.stderr
