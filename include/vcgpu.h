@@ -126,7 +126,8 @@ typedef struct {
     uint64_t donations;   /* subtrees put on the worklist */
     uint64_t steals;      /* subtrees taken from the worklist */
     float idle_fraction;  /* share of worker time spent waiting for work */
-    uint32_t reserved;
+    float active_ms;      /* device wall clock from the first worker's start to the
+                             last worker's exit (0 from a server) */
 } vc_launch_info;
 
 typedef struct vc_ctx vc_ctx;

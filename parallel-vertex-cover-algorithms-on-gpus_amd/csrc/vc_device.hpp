@@ -111,6 +111,7 @@ struct WorkerStats {
     uint64_t idle_ticks;  // device wall-clock ticks spent waiting for work
     uint64_t donations;   // subtrees put on the worklist
     uint64_t steals;      // subtrees taken from the worklist
+    uint64_t exit_tick;   // device wall clock when the worker exited
 };
 
 // Degree arrays are stored as DegT (int16 when every graph of the launch

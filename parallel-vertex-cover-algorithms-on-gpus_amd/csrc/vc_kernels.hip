@@ -27,6 +27,7 @@
 // on blockDim, src/helperFunctions.cuh:453-478).
 #include <hip/hip_runtime.h>
 
+#include "vcgpu.h"
 #include "vc_bound.hpp"
 #include "vc_device.hpp"
 
@@ -1214,6 +1215,13 @@ template <bool SERVE, class N>
 __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
 {
     Acquired r = {0u, kNone, 0u, kNone, 0u};
+    // A cancelled or late launch hands out no more work.  Checked on every
+    // call, not only when the worklist is empty: a worker that cycles
+    // through queued units shorter than kFlushEvery nodes never reaches the
+    // per-node check, and kept a launch running far past its time limit.
+    uint32_t late = 0;
+    if (lane_id() == 0) late = ld_rlx(&a.glob->cancel) || past_deadline(a);
+    if (uni(late)) return r;
     if (lane_id() == 0) fetch_add(&a.glob->counter, kWaitOne);
     uint32_t spins = 0;
     for (;;) {
@@ -1279,7 +1287,6 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
             return r;
         }
         if (uni(ld_rlx(&a.glob->done_flag)) || uni(ld_rlx(&a.glob->cancel))) return r;
-        uint32_t late = 0;
         if (lane_id() == 0) late = past_deadline(a);
         if (uni(late)) return r;
         backoff(++spins, a.idle_cap);
@@ -1568,6 +1575,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         ws.idle_ticks = idle_ticks;
         ws.donations = donations;
         ws.steals = steals;
+        ws.exit_tick = wall_clock64();
         a.wstats[wid] = ws;
         if constexpr (SERVE) {
             // a server's statistics and error word live in pinned host
@@ -1598,6 +1606,38 @@ __global__ void publish_kernel(Publish p)
     const uint32_t stride = gridDim.x * blockDim.x;
     for (int k = 0; k < 4; ++k)
         for (uint32_t i = tid; i < p.words[k]; i += stride) p.dst[k][i] = p.src[k][i];
+}
+
+// One graph's outcome from its search state after the launch (glob = the
+// launch's counters), usable on the host and on the device.
+__host__ __device__ inline vc_result result_of(const GState &s, uint32_t instance, const Global &glob,
+                                               uint32_t wall_khz)
+{
+    vc_result r;
+    r.minimum = 0;
+    r.k_found = 0;
+    r.solved = 0;
+    r.cover_from_gpu = 0;
+    r.tree_nodes = s.nodes;
+    r.done_us = 0;
+    const bool finished = s.done != 0u && !glob.error;
+    if (instance == VC_MVC) {
+        r.minimum = (uint32_t)s.best;
+        r.solved = finished && !s.budget_hit;
+        r.cover_from_gpu = s.cover_size >= 0 && s.cover_size == s.best;
+    } else {
+        r.k_found = s.found;
+        r.minimum = s.found ? (uint32_t)s.cover_size : 0u;
+        r.solved = finished && (s.found || !s.budget_hit);
+        r.cover_from_gpu = s.found != 0u;
+    }
+    r.status = glob.error ? VC_EDEVICE
+                          : (r.solved ? VC_OK : (glob.cancel && !glob.timed_out ? VC_ECANCELLED : VC_ETIMEOUT));
+    if (s.done && s.done_tick >= glob.t0 && glob.t0 != 0u && wall_khz > 0) {
+        const uint64_t us = (s.done_tick - glob.t0) * 1000u / (uint64_t)wall_khz;
+        r.done_us = (uint32_t)(us < 0xFFFFFFFFull ? us : 0xFFFFFFFFull);
+    }
+    return r;
 }
 
 __global__ void init_kernel(SolveArgs a, uint32_t units)
@@ -1854,7 +1894,7 @@ constexpr uint32_t kMaxBlockDim = 1024;   // the solve kernels' launch bound
 constexpr uint32_t kMaxWorkers = 0xFFFFu; // 16-bit worker counts in the termination word
 
 vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_params *p, Plan &pl, bool serve,
-                      bool off16_ok = false)
+                      bool off16_ok = false, size_t mem_budget = 0)
 {
     const uint32_t version = p->version;
     if (version != VC_HYBRID && version != VC_STACK_ONLY) return VC_EINVAL;
@@ -1941,7 +1981,7 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
     // Fit stacks + worklist into device memory; shrink the grid if needed
     // (the reference halves the worklist instead, src/auxFunctions.cpp:507-512).
     const size_t slot = pl.slot_bytes;
-    size_t budget = ctx->total_mem / 10 * 7;
+    const size_t budget = mem_budget ? mem_budget : ctx->total_mem / 10 * 7;
     for (;;) {
         pl.workers = pl.num_blocks * pl.wpb;
         size_t off = 0;
@@ -1969,7 +2009,8 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
     return VC_OK;
 }
 
-vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *p, Plan &pl)
+vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *p, Plan &pl,
+                    size_t mem_budget = 0)
 {
     // Stack depth: the stack never holds more entries than the cover size of
     // the current node, which stays below max(U, k+1) (prune at d >= U for
@@ -1982,7 +2023,7 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
     }
     bool off16_ok = true;
     for (int i = 0; i < set->count; ++i) off16_ok &= 2ull * set->h_desc[i].m < 65536ull;
-    vc_status st = plan_launch(ctx, set->nmax, need + 1u, p, pl, false, off16_ok);
+    vc_status st = plan_launch(ctx, set->nmax, need + 1u, p, pl, false, off16_ok, mem_budget);
     if (st != VC_OK) return st;
     pl.roots = (uint32_t)set->count * pl.units;
     return VC_OK;
@@ -2213,19 +2254,22 @@ vc_status vc_occupancy(vc_ctx *ctx, const vc_graphs *set, const vc_params *param
     return VC_OK;
 }
 
-vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, const vc_params *params,
-                   vc_result *results, uint8_t *const *covers, vc_launch_info *info)
+}  // extern "C"
+
+namespace {
+
+// Checks the jobs, plans the launch within mem_budget bytes of scratch (0 =
+// 70% of device memory) and writes the jobs into the set's descriptors.
+vc_status prepare_solve(vc_ctx *ctx, vc_graphs *set, const vc_job *jobs, const vc_params *params, Plan &pl,
+                        size_t mem_budget)
 {
-    if (!ctx || !set_c || !jobs || !params || !results) return VC_EINVAL;
-    vc_graphs *set = const_cast<vc_graphs *>(set_c);
     if (!hip_ok(hipSetDevice(ctx->device))) return VC_EDEVICE;
     for (int i = 0; i < set->count; ++i) {
         const vc_job &j = jobs[i];
         if (j.instance != VC_MVC && j.instance != VC_PVC) return VC_EINVAL;
         if (j.upper_bound > set->h_desc[i].n) return VC_EINVAL;
     }
-    Plan pl;
-    vc_status st = make_plan(ctx, set, jobs, params, pl);
+    vc_status st = make_plan(ctx, set, jobs, params, pl, mem_budget);
     if (st != VC_OK) return st;
 
     // job fields of the descriptors
@@ -2244,6 +2288,62 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     if (changed && !hip_ok(hipMemcpyAsync(set->d_desc, set->h_desc.data(), sizeof(GDesc) * set->count,
                                           hipMemcpyHostToDevice, ctx->stream)))
         return VC_EDEVICE;
+    return VC_OK;
+}
+
+// Launch arguments of a batch solve of `set` with scratch at base.
+SolveArgs solve_args(const vc_ctx *ctx, const vc_graphs *set, const vc_params *params, const Plan &pl, char *base)
+{
+    SolveArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.gbuf = set->d_words;
+    a.gdesc = set->d_desc;
+    a.gstate = set->d_state;
+    a.covers = set->d_covers;
+    a.num_graphs = (uint32_t)set->count;
+    // deadline on the device wall clock (every wait loop checks it)
+    const uint64_t ms = params->time_limit_ms ? params->time_limit_ms : 3600ull * 1000ull;
+    fill_args(a, pl, base, params->version, params->start_depth, ms * (uint64_t)ctx->wall_khz);
+    return a;
+}
+
+void fill_info(vc_launch_info *info, const vc_ctx *ctx, const Plan &pl, float ms, uint64_t nodes, uint64_t idle,
+               uint64_t donations, uint64_t steals, uint64_t t0, uint64_t last_exit)
+{
+    if (!info) return;
+    std::memset(info, 0, sizeof(*info));
+    info->num_blocks = pl.num_blocks;
+    info->block_dim = pl.block_dim;
+    info->workers = pl.workers;
+    info->blocks_per_cu = pl.blocks_per_cu;
+    info->use_global_memory = pl.variant != kLds16;
+    info->list_size = pl.qsize;
+    info->list_threshold = pl.qthreshold;
+    info->stack_depth = pl.depth;
+    info->kernel_ms = ms;
+    info->cu_count = (uint32_t)ctx->cu_count;
+    info->tree_nodes = nodes;
+    info->donations = donations;
+    info->steals = steals;
+    // share of worker time spent waiting for work
+    const double span = (double)ms * 1e-3 * (double)ctx->wall_khz * 1e3 * (double)pl.workers;
+    info->idle_fraction = span > 0 ? (float)std::min(1.0, (double)idle / span) : 0.f;
+    info->active_ms = (t0 && last_exit > t0 && ctx->wall_khz > 0) ? (float)((double)(last_exit - t0) / ctx->wall_khz)
+                                                                 : 0.f;
+}
+
+}  // namespace
+
+extern "C" {
+
+vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, const vc_params *params,
+                   vc_result *results, uint8_t *const *covers, vc_launch_info *info)
+{
+    if (!ctx || !set_c || !jobs || !params || !results) return VC_EINVAL;
+    vc_graphs *set = const_cast<vc_graphs *>(set_c);
+    Plan pl;
+    vc_status st = prepare_solve(ctx, set, jobs, params, pl, 0);
+    if (st != VC_OK) return st;
 
     if (ctx->scratch_bytes < pl.total) {
         ctx_release(ctx, ctx->scratch, false);
@@ -2252,19 +2352,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         if (!hip_ok(hipMalloc(&ctx->scratch, pl.total))) return VC_ENOMEM;
         ctx->scratch_bytes = pl.total;
     }
-    char *base = static_cast<char *>(ctx->scratch);
-    SolveArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.gbuf = set->d_words;
-    a.gdesc = set->d_desc;
-    a.gstate = set->d_state;
-    a.covers = set->d_covers;
-    a.num_graphs = (uint32_t)set->count;
-    {
-        // deadline on the device wall clock (every wait loop checks it)
-        uint64_t ms = params->time_limit_ms ? params->time_limit_ms : 3600ull * 1000ull;
-        fill_args(a, pl, base, params->version, params->start_depth, ms * (uint64_t)ctx->wall_khz);
-    }
+    const SolveArgs a = solve_args(ctx, set, params, pl, static_cast<char *>(ctx->scratch));
 
     bool want_covers = false;
     if (covers)
@@ -2366,59 +2454,25 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     const vcg::WorkerStats *wn = set->h_ws;
     const uint32_t *bitmaps = set->h_cov;
 
-    uint64_t total_nodes = 0, idle = 0, donations = 0, steals = 0;
+    uint64_t total_nodes = 0, idle = 0, donations = 0, steals = 0, last_exit = 0;
     for (uint32_t wi = 0; wi < pl.workers; ++wi) {
         const vcg::WorkerStats &x = wn[wi];
         total_nodes += x.nodes;
         idle += x.idle_ticks;
         donations += x.donations;
         steals += x.steals;
+        last_exit = std::max<uint64_t>(last_exit, x.exit_tick);
     }
     vc_status overall = hg.error ? VC_EDEVICE : (hg.timed_out ? VC_ETIMEOUT : VC_OK);
     for (int i = 0; i < set->count; ++i) {
-        const GState &s = hs[i];
-        const vc_job &j = jobs[i];
         vc_result &r = results[i];
-        std::memset(&r, 0, sizeof(r));
-        r.tree_nodes = s.nodes;
-        bool finished = s.done != 0u && !hg.error;
-        if (j.instance == VC_MVC) {
-            r.minimum = (uint32_t)s.best;
-            r.solved = finished && !s.budget_hit;
-            r.cover_from_gpu = s.cover_size >= 0 && s.cover_size == s.best;
-        } else {
-            r.k_found = s.found;
-            r.minimum = s.found ? (uint32_t)s.cover_size : 0u;
-            r.solved = finished && (s.found || !s.budget_hit);
-            r.cover_from_gpu = s.found != 0u;
-        }
-        r.status = hg.error ? VC_EDEVICE : (r.solved ? VC_OK : (hg.cancel && !hg.timed_out ? VC_ECANCELLED : VC_ETIMEOUT));
-        if (s.done && s.done_tick >= hg.t0 && hg.t0 != 0u && ctx->wall_khz > 0)
-            r.done_us = (uint32_t)std::min<uint64_t>((s.done_tick - hg.t0) * 1000u / (uint64_t)ctx->wall_khz,
-                                                     0xFFFFFFFFull);
+        r = vcg::result_of(hs[i], jobs[i].instance, hg, (uint32_t)ctx->wall_khz);
         if (covers && covers[i] && r.cover_from_gpu) {
             const uint32_t *bm = &bitmaps[set->h_desc[i].off_cover];
             for (uint32_t v = 0; v < set->h_desc[i].n; ++v) covers[i][v] = (bm[v >> 5] >> (v & 31u)) & 1u;
         }
     }
-    if (info) {
-        info->num_blocks = pl.num_blocks;
-        info->block_dim = pl.block_dim;
-        info->workers = pl.workers;
-        info->blocks_per_cu = pl.blocks_per_cu;
-        info->use_global_memory = pl.variant != kLds16;
-        info->list_size = pl.qsize;
-        info->list_threshold = pl.qthreshold;
-        info->stack_depth = pl.depth;
-        info->kernel_ms = ms;
-        info->cu_count = (uint32_t)ctx->cu_count;
-        info->tree_nodes = total_nodes;
-        info->donations = donations;
-        info->steals = steals;
-        // share of worker time spent waiting for work
-        double span = (double)ms * 1e-3 * (double)ctx->wall_khz * 1e3 * (double)pl.workers;
-        info->idle_fraction = span > 0 ? (float)std::min(1.0, (double)idle / span) : 0.f;
-    }
+    fill_info(info, ctx, pl, ms, total_nodes, idle, donations, steals, hg.t0, last_exit);
     return overall;
 }
 

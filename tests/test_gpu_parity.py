@@ -222,24 +222,61 @@ def test_reference_known_answers_pvc(ctx, case):
     assert O.cover_is_valid(og, np.asarray(yes.cover, np.uint8)) and int(np.sum(yes.cover)) <= opt, name
 
 
-@pytest.mark.parametrize("case", [k for k in _known() if not k[3]["cpu_confirmed"]], ids=lambda k: k[0])
+# Graphs of the same table (test/runGraphs.py:8-30) that the CPU oracle cannot
+# finish in minutes, with the device time limit (ms) within which this solver
+# proves the known MVC on one MI355X.  cop_hat500-3 and cop_hat700-2 are two of
+# the reference harness's own benchmark graphs (RUN_GRAPHS, test/runGraphs.py:44-47).
+GPU_SOLVES = {
+    "Power-Grid": 30000,
+    "cop_hat300-3": 30000,
+    "cop_hat500-2": 30000,
+    "cop_hat700-2": 30000,
+    "cop_hat500-3": 150000,
+    "formated_wikipedia_link_lo": 150000,
+}
+
+
+@pytest.mark.parametrize("case", [k for k in _known() if not k[3]["cpu_confirmed"] and k[0] in GPU_SOLVES],
+                         ids=lambda k: k[0])
 def test_reference_known_answers_hard(ctx, case):
-    """Graphs of the same table the CPU oracle cannot finish in minutes: with
-    a 30 s limit the device incumbent must never go below the known MVC, its
-    cover must be valid, and a finished search must hit the known value."""
+    """The known MVC proven on the device: the search finishes within its
+    time limit with the reference's value and a valid cover of that size."""
     name, n, e, meta = case
     g = vcgpu.csr_from_edges(n, e)
     og = O.from_edges(n, e)
     U, ucov = vcgpu.upper_bound(g, seed=1)
     gs = ctx.upload(g)
-    (r,), info, st = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), P(time_limit_ms=30000), raise_on_error=False)
+    (r,), info, st = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), P(time_limit_ms=GPU_SOLVES[name]),
+                               raise_on_error=False)
+    print(f"{name}: solved={r.solved} minimum={r.minimum} known={meta['mvc']} nodes={r.tree_nodes} "
+          f"kernel_ms={info.kernel_ms:.0f}")
+    assert r.solved and st == vcgpu.VC_OK, (name, st)
+    assert r.minimum == meta["mvc"], (name, r.minimum, meta["mvc"])
+    cover = r.cover if r.cover is not None else ucov
+    assert O.cover_is_valid(og, np.asarray(cover, np.uint8)) and int(np.sum(cover)) == r.minimum
+
+
+@pytest.mark.parametrize("case", [k for k in _known() if not k[3]["cpu_confirmed"] and k[0] not in GPU_SOLVES],
+                         ids=lambda k: k[0])
+def test_reference_known_answers_out_of_reach(ctx, case):
+    """The table's graphs this rule set does not finish (cop_hat1000-2, and the
+    PACE graphs vc-exact_009 / vc-exact_023 whose int32 node state lives in
+    HBM): within a 10 s limit the device incumbent never goes below the known
+    MVC, its cover is valid, and the launch ends on time."""
+    name, n, e, meta = case
+    g = vcgpu.csr_from_edges(n, e)
+    og = O.from_edges(n, e)
+    U, ucov = vcgpu.upper_bound(g, seed=1)
+    gs = ctx.upload(g)
+    (r,), info, st = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), P(time_limit_ms=10000), raise_on_error=False)
+    print(f"{name}: solved={r.solved} minimum={r.minimum} known={meta['mvc']} U={U} nodes={r.tree_nodes} "
+          f"kernel_ms={info.kernel_ms:.0f} workers={info.workers}")
     assert r.minimum >= meta["mvc"], (name, r.minimum)
     cover = r.cover if r.cover is not None else ucov
     assert O.cover_is_valid(og, np.asarray(cover, np.uint8)) and int(np.sum(cover)) == r.minimum
     if r.solved:
         assert r.minimum == meta["mvc"]
-    print(f"{name}: solved={r.solved} minimum={r.minimum} known={meta['mvc']} nodes={r.tree_nodes} "
-          f"kernel_ms={info.kernel_ms:.0f}")
+    assert info.kernel_ms < 10000 + 2000, (name, info.kernel_ms)  # cancellation reaches every worker
 
 
 def test_graph_captured_dispatch_replays_match_eager(ctx):
