@@ -159,6 +159,25 @@ vc_status vc_graphs_free(vc_ctx *ctx, vc_graphs *set);
 vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *params,
                    vc_result *results, uint8_t *const *covers, vc_launch_info *info);
 
+/* Allocation-free variant of vc_solve.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and covers as vc_solve; it calls no hipMalloc, hipFree or
+ * hipHostMalloc.  Its launch scratch (stacks, worklist, counters, worker
+ * statistics, read-back) is one fixed device array of 128 MiB per GPU, which
+ * the library holds from the moment it is loaded on that GPU until the process
+ * ends.  The grid shrinks until the stacks fit in it; a set whose smallest
+ * launch (one workgroup with exact-depth stacks) does not fit returns
+ * VC_ENOMEM, as does a set whose read-back (32 bytes per graph plus one byte
+ * per vertex) alone would fill it.  Every call on one GPU shares that array
+ * and leaves its last contents there (each call clears what must start at
+ * zero); calls that overlap on one GPU race, so the library serialises its
+ * own calls per GPU with a host lock (each process has its own array).
+ * VC_FLAG_GRAPH is refused with VC_ENOTSUP.  Results return by
+ * device-to-host copies into results[] and covers[i]. */
+vc_status vc_solve_noalloc(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const vc_params *params,
+                           vc_result *results, uint8_t *const *covers, vc_launch_info *info);
+
 /* Workgroups per CU the solve would run with (replaces
  * cudaOccupancyMaxActiveBlocksPerMultiprocessor, src/main.cu:131-191). */
 vc_status vc_occupancy(vc_ctx *ctx, const vc_graphs *set, const vc_params *params, int *blocks_per_cu,

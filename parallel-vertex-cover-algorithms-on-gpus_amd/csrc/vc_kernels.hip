@@ -1609,7 +1609,8 @@ __global__ void publish_kernel(Publish p)
 }
 
 // One graph's outcome from its search state after the launch (glob = the
-// launch's counters), usable on the host and on the device.
+// launch's counters), for the host read-back of vc_solve and the device
+// read-back of vc_solve_noalloc alike.
 __host__ __device__ inline vc_result result_of(const GState &s, uint32_t instance, const Global &glob,
                                                uint32_t wall_khz)
 {
@@ -1659,6 +1660,66 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
     }
 }
 
+// Read-back of vc_solve_noalloc, written inside its fixed scratch: launch
+// totals, one vc_result per graph, and every cover bitmap expanded to one
+// byte per vertex (graph g's bytes start at 32 x its first bitmap word).
+struct NoAllocHead {
+    unsigned long long nodes, idle_ticks, donations, steals, last_exit, t0;
+    uint32_t error, timed_out, cancel, pad;
+};
+static_assert(sizeof(NoAllocHead) == 64, "NoAllocHead is one 64-byte line");
+
+// One workgroup of 256 threads sums the worker statistics; every thread
+// converts results and expands cover words.
+__global__ __launch_bounds__(256) void noalloc_results_kernel(SolveArgs a, uint32_t wall_khz, uint32_t cover_words,
+                                                              NoAllocHead *head, vc_result *out, uint8_t *cover_bytes)
+{
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t g = tid; g < a.num_graphs; g += stride)
+        out[g] = result_of(a.gstate[g], a.gdesc[g].instance, *a.glob, wall_khz);
+    for (uint32_t i = tid; i < cover_words * 32u; i += stride)
+        cover_bytes[i] = (uint8_t)((a.covers[i >> 5] >> (i & 31u)) & 1u);
+    if (blockIdx.x != 0) return;
+    __shared__ unsigned long long part[5][256];
+    unsigned long long s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+    for (uint32_t w = threadIdx.x; w < a.workers; w += 256u) {
+        const WorkerStats &x = a.wstats[w];
+        s0 += x.nodes;
+        s1 += x.idle_ticks;
+        s2 += x.donations;
+        s3 += x.steals;
+        s4 = x.exit_tick > s4 ? x.exit_tick : s4;
+    }
+    part[0][threadIdx.x] = s0;
+    part[1][threadIdx.x] = s1;
+    part[2][threadIdx.x] = s2;
+    part[3][threadIdx.x] = s3;
+    part[4][threadIdx.x] = s4;
+    __syncthreads();
+    for (uint32_t h = 128u; h > 0u; h >>= 1) {
+        if (threadIdx.x < h) {
+            for (int k = 0; k < 4; ++k) part[k][threadIdx.x] += part[k][threadIdx.x + h];
+            const unsigned long long o = part[4][threadIdx.x + h];
+            if (o > part[4][threadIdx.x]) part[4][threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        NoAllocHead r = {};
+        r.nodes = part[0][0];
+        r.idle_ticks = part[1][0];
+        r.donations = part[2][0];
+        r.steals = part[3][0];
+        r.last_exit = part[4][0];
+        r.t0 = a.glob->t0;
+        r.error = a.glob->error;
+        r.timed_out = a.glob->timed_out;
+        r.cancel = a.glob->cancel;
+        *head = r;
+    }
+}
+
 template <typename DegT, bool HBM, bool SERVE, bool OFF16 = false>
 __global__ __launch_bounds__(1024) void solve_kernel(SolveArgs a)
 {
@@ -1682,6 +1743,15 @@ template __global__ void solve_kernel<int16_t, true, true>(SolveArgs);
 template __global__ void solve_kernel<int32_t, true, true>(SolveArgs);
 
 }  // namespace vcg
+
+// Launch scratch of vc_solve_noalloc: stacks, worklist, counters, worker
+// statistics and the read-back area, in device memory that the library
+// holds from the moment its code is loaded on a GPU until the process ends
+// (one copy per GPU, at an address that hipGetSymbolAddress gives per
+// device).  Host code never passes this name to a kernel or a copy: its
+// host-side symbol is not a device address.
+constexpr size_t kNoAllocScratchBytes = 128ull << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
 
 // ===========================================================================
 // host launcher and C ABI (include/vcgpu.h)
@@ -1757,6 +1827,32 @@ void ctx_flush_deferred(vc_ctx *ctx)
     }
     for (auto &x : d) free_now(x.first, x.second);
 }
+
+namespace {
+
+// vc_solve_noalloc's scratch per device: the device address of `scratch` on
+// each GPU, taken with that device current (when the first context on the
+// device is created, else at the first vc_solve_noalloc call there), and a
+// lock that serialises this process's calls sharing that GPU's array.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices] = {};
+std::mutex g_noalloc_call_mu[kMaxDevices];
+
+// The calling thread's current device must be `device`.  Returns the
+// device address of that GPU's scratch, or nullptr if it cannot be found.
+void *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess && p) g_noalloc_ptr[device] = p;
+    }
+    return g_noalloc_ptr[device];
+}
+
+}  // namespace
 
 // What a captured dispatch depends on; a different value means re-capture.
 struct DispatchKey {
@@ -2132,6 +2228,9 @@ vc_status vc_ctx_create(int device, vc_ctx **out)
         delete c;
         return VC_EDEVICE;
     }
+    // the device address of vc_solve_noalloc's fixed scratch on this GPU
+    // (a failure here is reported by that entry point, which tries again)
+    (void)noalloc_scratch(device);
     *out = c;
     return VC_OK;
 }
@@ -2474,6 +2573,64 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
     }
     fill_info(info, ctx, pl, ms, total_nodes, idle, donations, steals, hg.t0, last_exit);
     return overall;
+}
+
+vc_status vc_solve_noalloc(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, const vc_params *params,
+                           vc_result *results, uint8_t *const *covers, vc_launch_info *info)
+{
+    if (!ctx || !set_c || !jobs || !params || !results) return VC_EINVAL;
+    if (params->flags & VC_FLAG_GRAPH) return VC_ENOTSUP;  // a captured dispatch allocates its graph
+    vc_graphs *set = const_cast<vc_graphs *>(set_c);
+    if (!hip_ok(hipSetDevice(ctx->device))) return VC_EDEVICE;
+    char *const base = static_cast<char *>(noalloc_scratch(ctx->device));
+    if (!base) return VC_EDEVICE;
+    // read-back area at the end of the scratch; the launch plan gets the rest
+    // and shrinks its grid to fit (VC_ENOMEM if one workgroup does not)
+    auto up256 = [](size_t x) { return (x + 255u) / 256u * 256u; };
+    const size_t res_bytes = up256(sizeof(vc_result) * (size_t)set->count);
+    const size_t cov_bytes = up256((size_t)set->cover_words * 32u);
+    const size_t tail = sizeof(vcg::NoAllocHead) + 192u + res_bytes + cov_bytes;
+    if (tail >= kNoAllocScratchBytes) return VC_ENOMEM;
+    const size_t tail_at = kNoAllocScratchBytes - tail;  // a multiple of 256
+    std::lock_guard<std::mutex> lk(g_noalloc_call_mu[ctx->device]);
+    Plan pl;
+    vc_status st = prepare_solve(ctx, set, jobs, params, pl, tail_at);
+    if (st != VC_OK) return st;
+    if (pl.total > tail_at) return VC_ENOMEM;
+    const SolveArgs a = solve_args(ctx, set, params, pl, base);
+    auto *d_head = reinterpret_cast<vcg::NoAllocHead *>(base + tail_at);
+    auto *d_res = reinterpret_cast<vc_result *>(base + tail_at + 256u);
+    auto *d_cov = reinterpret_cast<uint8_t *>(base + tail_at + 256u + res_bytes);
+
+    hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, ctx->stream, a, pl.units);
+    if (!hip_ok(hipGetLastError()) || !hip_ok(hipEventRecord(ctx->ev0, ctx->stream)) ||
+        !launch_solve(pl, ctx->stream, a) || !hip_ok(hipEventRecord(ctx->ev1, ctx->stream)))
+        return VC_EDEVICE;
+    hipLaunchKernelGGL(vcg::noalloc_results_kernel, dim3(64), dim3(256), 0, ctx->stream, a,
+                       (uint32_t)ctx->wall_khz, set->cover_words, d_head, d_res, d_cov);
+    vcg::NoAllocHead head;
+    if (!hip_ok(hipGetLastError()) ||
+        !hip_ok(hipMemcpyAsync(&head, d_head, sizeof(head), hipMemcpyDeviceToHost, ctx->stream)) ||
+        !hip_ok(hipMemcpyAsync(results, d_res, sizeof(vc_result) * (size_t)set->count, hipMemcpyDeviceToHost,
+                               ctx->stream)) ||
+        !hip_ok(hipEventRecord(ctx->ev2, ctx->stream)) || !hip_ok(hipEventSynchronize(ctx->ev2)))
+        return VC_EDEVICE;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) ms = 0.f;
+    if (covers) {
+        bool any = false;
+        for (int i = 0; i < set->count; ++i) {
+            if (!covers[i] || !results[i].cover_from_gpu || set->h_desc[i].n == 0) continue;
+            if (!hip_ok(hipMemcpyAsync(covers[i], d_cov + (size_t)set->h_desc[i].off_cover * 32u, set->h_desc[i].n,
+                                       hipMemcpyDeviceToHost, ctx->stream)))
+                return VC_EDEVICE;
+            any = true;
+        }
+        if (any && (!hip_ok(hipEventRecord(ctx->ev2, ctx->stream)) || !hip_ok(hipEventSynchronize(ctx->ev2))))
+            return VC_EDEVICE;
+    }
+    fill_info(info, ctx, pl, ms, head.nodes, head.idle_ticks, head.donations, head.steals, head.t0, head.last_exit);
+    return head.error ? VC_EDEVICE : (head.timed_out ? VC_ETIMEOUT : VC_OK);
 }
 
 }  // extern "C"

@@ -75,7 +75,7 @@ EXPORTS = [
     "vc_ctx_create", "vc_ctx_destroy", "vc_graphs_upload", "vc_graphs_free", "vc_solve", "vc_occupancy",
     "vc_load_graph_file", "vc_load_graph_file_indexed", "vc_csr_from_edges", "vc_csr_free", "vc_check_graph",
     "vc_approx_remove_max", "vc_approx_remove_edge", "vc_status_string", "vc_server_start", "vc_server_submit", "vc_server_poll",
-    "vc_server_wait", "vc_server_stop", "vc_now_ns", "vc_autotune",
+    "vc_server_wait", "vc_server_stop", "vc_now_ns", "vc_autotune", "vc_solve_noalloc",
 ]
 VC_EAGAIN = 8
 
@@ -103,6 +103,8 @@ def lib():
         L.vc_graphs_free.argtypes = [C.c_void_p, C.c_void_p]
         L.vc_solve.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_Job), C.POINTER(_Params), C.POINTER(_Result),
                                C.POINTER(_u8p), C.POINTER(_Info)]
+        L.vc_solve_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_Job), C.POINTER(_Params),
+                                       C.POINTER(_Result), C.POINTER(_u8p), C.POINTER(_Info)]
         L.vc_occupancy.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_Params), _i32p, _i32p, _i32p]
         L.vc_autotune.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(_Job), C.POINTER(_Params), C.POINTER(_Params),
                                   C.POINTER(C.c_float)]
@@ -354,8 +356,10 @@ class Context:
         return Params._from_c(out), float(ms.value)
 
     def solve(self, gset: GraphSet, jobs, params: Params | None = None, want_covers: bool = True,
-              raise_on_error: bool = True):
-        """One persistent launch over every graph of the set.
+              raise_on_error: bool = True, noalloc: bool = False):
+        """One persistent launch over every graph of the set (vc_solve, or
+        vc_solve_noalloc with noalloc=True: launch scratch in the library's
+        fixed device array, no allocation).
         Returns (list[Result], LaunchInfo, overall_status)."""
         G = len(gset)
         if isinstance(jobs, Job):
@@ -366,9 +370,10 @@ class Context:
         p = (params or Params())._c()
         bufs = [np.zeros(max(g.n, 1), np.uint8) for g in gset.graphs] if want_covers else None
         cov = (_u8p * G)(*[b.ctypes.data_as(_u8p) for b in bufs]) if want_covers else None
-        st = lib().vc_solve(self.handle, gset.handle, cj, C.byref(p), res, cov, C.byref(info))
+        fn = lib().vc_solve_noalloc if noalloc else lib().vc_solve
+        st = fn(self.handle, gset.handle, cj, C.byref(p), res, cov, C.byref(info))
         if st != VC_OK and raise_on_error:
-            raise VCError(st, "vc_solve")
+            raise VCError(st, "vc_solve_noalloc" if noalloc else "vc_solve")
         out = []
         for i in range(G):
             r = res[i]
