@@ -128,6 +128,10 @@ typedef struct {
     float idle_fraction;  /* share of worker time spent waiting for work */
     float active_ms;      /* device wall clock from the first worker's start to the
                              last worker's exit (0 from a server) */
+    uint64_t state_bytes; /* HBM bytes of node state the search moved: every stack
+                             push and pop, worklist steal, and (twice) donation copies
+                             the graph's degree/sketch records, 8 bytes per vertex (16
+                             in the int32 layout) in 16-byte vectors; 0 from a server */
 } vc_launch_info;
 
 typedef struct vc_ctx vc_ctx;

@@ -69,7 +69,8 @@ struct Global {
     uint32_t cancel;          // cancel request (host, or the deadline)
     uint32_t timed_out;       // the deadline passed before the search finished
     uint64_t t0;              // wall clock when the first solve worker started (0 = none yet)
-    uint32_t pad5[10];
+    unsigned long long state_bytes;  // HBM bytes of node-state slot copies (stacks, worklist)
+    uint32_t pad5[8];
     // serving: device copy of the host's request tail / shutdown flag, and the
     // time of the last host poll (polls are rate-limited to one per interval)
     uint32_t host_tail;

@@ -914,6 +914,15 @@ __device__ __forceinline__ void copy_state(char *__restrict__ dst, const char *_
     }
 }
 
+// HBM bytes one copy_state of cnt vertices reads or writes on the slot side
+// (16-byte vectors: two 4-byte arrays packed, or 4 + 4 + 8 bytes per vertex).
+template <class N>
+__device__ __forceinline__ uint32_t copy_bytes(uint32_t cnt)
+{
+    const uint32_t q = (cnt + 3u) >> 2;
+    return N::kPacked ? 32u * q : 64u * q;
+}
+
 // ---------------------------------------------------------------------------
 // broker worklist (src/BWDWorkList.cuh) with 64-bit ring positions
 // ---------------------------------------------------------------------------
@@ -1081,8 +1090,15 @@ __device__ void record_cover(const SolveArgs &a, const N &w, const GDesc &gd, ui
     }
 }
 
-__device__ __forceinline__ void flush_nodes(const SolveArgs &a, const GDesc &gd, uint32_t g, uint64_t &pending)
+// Adds the nodes spent on graph g since the last flush to its count (and
+// stops it at its budget), and the HBM bytes of the slot copies made for it
+// (bytes_per_copy each) to the launch total.
+__device__ __forceinline__ void flush_nodes(const SolveArgs &a, const GDesc &gd, uint32_t g, uint64_t &pending,
+                                            uint32_t &copies, uint32_t bytes_per_copy)
 {
+    if (lane_id() == 0 && copies != 0u)
+        fetch_add(&a.glob->state_bytes, (unsigned long long)copies * bytes_per_copy);
+    copies = 0;
     if (pending == 0) return;
     if (lane_id() == 0) {
         GState *gs = &a.gstate[g];
@@ -1384,6 +1400,10 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
     uint32_t off16_g = kNone;  // graph whose offsets the LDS copy holds
     bool full = true;   // the next leaf pass must scan every vertex
     uint64_t pending = 0, total_nodes = 0, idle_ticks = 0, donations = 0, steals = 0;
+    // slot copies of the current graph's node state since the last flush:
+    // stack pushes and pops, steals (ring slot in) and donations (stack slot
+    // to ring slot, two); flush_nodes adds their HBM bytes to the launch total
+    uint32_t copies = 0;
 
     for (;;) {
         if (!have) {
@@ -1397,6 +1417,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 const uint32_t ts = (uint32_t)top % a.depth;
                 const uint2 m = stk_meta[ts];  // in flight during the copy
                 copy_state<NodeT>(w.base, stk + (size_t)ts * slot_bytes, a.npad, w.n);
+                ++copies;
                 w.sync();
                 d = uni(m.x);
                 uint32_t v = uni(m.y);
@@ -1408,7 +1429,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
             } else {
                 if (g != kNone) {
                     total_nodes += pending;
-                    flush_nodes(a, gd, g, pending);
+                    flush_nodes(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n));
                     finish_unit<SERVE>(a, g, gd);
                     g = kNone;
                 }
@@ -1420,6 +1441,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 steals += q.kind == 1;
                 if (q.kind == 1) {
                     g = q.g;
+                    copies = 1u;
                 } else {
                     g = a.version == 1 ? q.root : (q.root >> a.start_depth);
                 }
@@ -1468,7 +1490,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         }
         if (++pending >= kFlushEvery) {
             total_nodes += pending;
-            flush_nodes(a, gd, g, pending);
+            flush_nodes(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n));
             uint32_t stop_all = 0;
             if (lane_id() == 0) stop_all = ld_rlx(&a.glob->cancel) || past_deadline(a);
             if (uni(stop_all)) {
@@ -1526,6 +1548,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                     if (try_enqueue(a, w, stk + (size_t)bs * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
                         ++bot;
                         ++donations;
+                        copies += 2u;
                         moved = true;
                         break;
                     }
@@ -1548,6 +1571,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         const uint32_t ts = (uint32_t)top % a.depth;
         copy_state<NodeT>(stk + (size_t)ts * slot_bytes, w.base, a.npad, w.n);
         if (lane_id() == 0) stk_meta[ts] = make_uint2(d, v);
+        ++copies;
         if (a.version == 1) {
             // Donate while workers wait and the worklist holds fewer items
             // than there are waiting workers (and fewer than the -t
@@ -1563,6 +1587,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 if (try_enqueue(a, w, stk + (size_t)bs * slot_bytes, g, uni(m.x), uni(m.y), slot_bytes)) {
                     ++bot;
                     ++donations;
+                    copies += 2u;
                 }
             }
         }
@@ -1664,10 +1689,10 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
 // totals, one vc_result per graph, and every cover bitmap expanded to one
 // byte per vertex (graph g's bytes start at 32 x its first bitmap word).
 struct NoAllocHead {
-    unsigned long long nodes, idle_ticks, donations, steals, last_exit, t0;
-    uint32_t error, timed_out, cancel, pad;
+    unsigned long long nodes, idle_ticks, donations, steals, last_exit, t0, state_bytes;
+    uint32_t error, timed_out, cancel, pad[15];
 };
-static_assert(sizeof(NoAllocHead) == 64, "NoAllocHead is one 64-byte line");
+static_assert(sizeof(NoAllocHead) == 128, "NoAllocHead is two 64-byte lines");
 
 // One workgroup of 256 threads sums the worker statistics; every thread
 // converts results and expands cover words.
@@ -1681,27 +1706,24 @@ __global__ __launch_bounds__(256) void noalloc_results_kernel(SolveArgs a, uint3
     for (uint32_t i = tid; i < cover_words * 32u; i += stride)
         cover_bytes[i] = (uint8_t)((a.covers[i >> 5] >> (i & 31u)) & 1u);
     if (blockIdx.x != 0) return;
-    __shared__ unsigned long long part[5][256];
-    unsigned long long s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+    // sums of nodes, idle ticks, donations, steals, state bytes; max of exit ticks
+    __shared__ unsigned long long part[6][256];
+    unsigned long long acc[6] = {0, 0, 0, 0, 0, 0};
     for (uint32_t w = threadIdx.x; w < a.workers; w += 256u) {
         const WorkerStats &x = a.wstats[w];
-        s0 += x.nodes;
-        s1 += x.idle_ticks;
-        s2 += x.donations;
-        s3 += x.steals;
-        s4 = x.exit_tick > s4 ? x.exit_tick : s4;
+        acc[0] += x.nodes;
+        acc[1] += x.idle_ticks;
+        acc[2] += x.donations;
+        acc[3] += x.steals;
+        acc[5] = x.exit_tick > acc[5] ? x.exit_tick : acc[5];
     }
-    part[0][threadIdx.x] = s0;
-    part[1][threadIdx.x] = s1;
-    part[2][threadIdx.x] = s2;
-    part[3][threadIdx.x] = s3;
-    part[4][threadIdx.x] = s4;
+    for (int k = 0; k < 6; ++k) part[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (uint32_t h = 128u; h > 0u; h >>= 1) {
         if (threadIdx.x < h) {
-            for (int k = 0; k < 4; ++k) part[k][threadIdx.x] += part[k][threadIdx.x + h];
-            const unsigned long long o = part[4][threadIdx.x + h];
-            if (o > part[4][threadIdx.x]) part[4][threadIdx.x] = o;
+            for (int k = 0; k < 5; ++k) part[k][threadIdx.x] += part[k][threadIdx.x + h];
+            const unsigned long long o = part[5][threadIdx.x + h];
+            if (o > part[5][threadIdx.x]) part[5][threadIdx.x] = o;
         }
         __syncthreads();
     }
@@ -1711,7 +1733,8 @@ __global__ __launch_bounds__(256) void noalloc_results_kernel(SolveArgs a, uint3
         r.idle_ticks = part[1][0];
         r.donations = part[2][0];
         r.steals = part[3][0];
-        r.last_exit = part[4][0];
+        r.state_bytes = a.glob->state_bytes;
+        r.last_exit = part[5][0];
         r.t0 = a.glob->t0;
         r.error = a.glob->error;
         r.timed_out = a.glob->timed_out;
@@ -2407,7 +2430,7 @@ SolveArgs solve_args(const vc_ctx *ctx, const vc_graphs *set, const vc_params *p
 }
 
 void fill_info(vc_launch_info *info, const vc_ctx *ctx, const Plan &pl, float ms, uint64_t nodes, uint64_t idle,
-               uint64_t donations, uint64_t steals, uint64_t t0, uint64_t last_exit)
+               uint64_t donations, uint64_t steals, uint64_t t0, uint64_t last_exit, uint64_t state_bytes)
 {
     if (!info) return;
     std::memset(info, 0, sizeof(*info));
@@ -2429,6 +2452,7 @@ void fill_info(vc_launch_info *info, const vc_ctx *ctx, const Plan &pl, float ms
     info->idle_fraction = span > 0 ? (float)std::min(1.0, (double)idle / span) : 0.f;
     info->active_ms = (t0 && last_exit > t0 && ctx->wall_khz > 0) ? (float)((double)(last_exit - t0) / ctx->wall_khz)
                                                                  : 0.f;
+    info->state_bytes = state_bytes;
 }
 
 }  // namespace
@@ -2571,7 +2595,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
             for (uint32_t v = 0; v < set->h_desc[i].n; ++v) covers[i][v] = (bm[v >> 5] >> (v & 31u)) & 1u;
         }
     }
-    fill_info(info, ctx, pl, ms, total_nodes, idle, donations, steals, hg.t0, last_exit);
+    fill_info(info, ctx, pl, ms, total_nodes, idle, donations, steals, hg.t0, last_exit, hg.state_bytes);
     return overall;
 }
 
@@ -2589,7 +2613,8 @@ vc_status vc_solve_noalloc(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jo
     auto up256 = [](size_t x) { return (x + 255u) / 256u * 256u; };
     const size_t res_bytes = up256(sizeof(vc_result) * (size_t)set->count);
     const size_t cov_bytes = up256((size_t)set->cover_words * 32u);
-    const size_t tail = sizeof(vcg::NoAllocHead) + 192u + res_bytes + cov_bytes;
+    static_assert(sizeof(vcg::NoAllocHead) <= 256u, "read-back header fits its 256 bytes");
+    const size_t tail = 256u + res_bytes + cov_bytes;
     if (tail >= kNoAllocScratchBytes) return VC_ENOMEM;
     const size_t tail_at = kNoAllocScratchBytes - tail;  // a multiple of 256
     std::lock_guard<std::mutex> lk(g_noalloc_call_mu[ctx->device]);
@@ -2629,7 +2654,8 @@ vc_status vc_solve_noalloc(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jo
         if (any && (!hip_ok(hipEventRecord(ctx->ev2, ctx->stream)) || !hip_ok(hipEventSynchronize(ctx->ev2))))
             return VC_EDEVICE;
     }
-    fill_info(info, ctx, pl, ms, head.nodes, head.idle_ticks, head.donations, head.steals, head.t0, head.last_exit);
+    fill_info(info, ctx, pl, ms, head.nodes, head.idle_ticks, head.donations, head.steals, head.t0, head.last_exit,
+              head.state_bytes);
     return head.error ? VC_EDEVICE : (head.timed_out ? VC_ETIMEOUT : VC_OK);
 }
 

@@ -61,7 +61,8 @@ class _Info(C.Structure):
                 ("blocks_per_cu", C.c_uint32), ("use_global_memory", C.c_uint32), ("list_size", C.c_uint32),
                 ("list_threshold", C.c_uint32), ("stack_depth", C.c_uint32), ("kernel_ms", C.c_float),
                 ("cu_count", C.c_uint32), ("tree_nodes", C.c_uint64), ("donations", C.c_uint64),
-                ("steals", C.c_uint64), ("idle_fraction", C.c_float), ("active_ms", C.c_float)]
+                ("steals", C.c_uint64), ("idle_fraction", C.c_float), ("active_ms", C.c_float),
+                ("state_bytes", C.c_uint64)]
 
 
 class _ServerConfig(C.Structure):
@@ -250,6 +251,7 @@ class LaunchInfo:
     steals: int
     idle_fraction: float
     active_ms: float  # first worker start to last worker exit (device wall clock)
+    state_bytes: int  # HBM bytes of node state moved by stack and worklist copies
 
 
 @dataclass
