@@ -133,6 +133,7 @@ struct SolveArgs {
     uint32_t num_graphs;
     uint32_t npad;          // slot length in DegT values
     uint32_t lcap;          // mark-list capacity per worker (multiple of 4)
+    uint32_t ncap;          // capacity of each rule-candidate list (multiple of 4)
     uint32_t mwords;        // mark-bitset words per worker (multiple of 4)
     uint32_t worker_bytes;  // bytes of per-worker node storage (LDS or HBM)
     uint32_t depth;         // stack slots per worker

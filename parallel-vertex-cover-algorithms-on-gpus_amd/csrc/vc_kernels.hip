@@ -230,6 +230,7 @@ template <typename DegT> constexpr uint32_t node_bytes_per_vertex()
 template <typename DegT, bool HBM, bool OFF16 = false>
 struct Node {
     static constexpr bool kPacked = sizeof(DegT) == 2;
+    static constexpr bool kHbm = HBM;
     using s2_t = typename SketchOf<DegT>::s2_t;
     using cand_t = typename SketchOf<DegT>::cand_t;
     char *base;          // start of the node state (what a slot copy moves)
@@ -237,7 +238,8 @@ struct Node {
     int32_t *deg32;      // [npad] degrees (int32 layout)
     uint32_t *s1w;       // [npad] s1 (int32 layout)
     s2_t *s2;            // [npad] sum of squared live neighbour ids
-    cand_t *cand;        // [ncap]: vertices whose degree fell to 1 since the last leaf pass
+    cand_t *cand;        // [ncap]: vertices whose degree fell 2 -> 1 since the last leaf pass
+    cand_t *cand3;       // [ncap]: vertices whose degree fell 3 -> 2 since the last triangle pass
     uint32_t ncap;
     uint32_t *mark;      // [mwords] bitset, all zero between passes
     cand_t *list;        // [lcap]: vertices marked in the current pass
@@ -413,21 +415,43 @@ __device__ __forceinline__ int deg_or_none(const N &w, uint32_t v, uint32_t n)
     return v < n ? d : -1;
 }
 
-// Leaf candidates: every vertex whose degree falls from 2 to 1 is appended
-// (lanes with t == kNone add nothing).  Degrees never grow along a path, so
-// a vertex enters at most once between two leaf passes; a count beyond ncap
-// (never expected) makes the next leaf pass scan every vertex.  Returns the
-// new count (wave-uniform; every lane must call).
-template <class N>
-__device__ __forceinline__ uint32_t add_leaf_cands(const N &w, uint32_t nc, uint32_t t)
+// Rule candidates of the current node.  A vertex can only become a leaf
+// (degree 1) or an unexamined degree-2 vertex through a decrement: the
+// decrements of every apply list each vertex whose degree falls 2 -> 1 (a
+// leaf-rule candidate) or 3 -> 2 (a triangle-rule candidate).  Degrees never
+// grow along a path, so a vertex enters each list at most once between two
+// passes of its rule.  A count beyond the list capacity makes that rule's
+// next pass scan every vertex, as does a root (full1 / full2).
+struct Cands {
+    uint32_t n1, n2;   // listed leaf / triangle candidates
+    bool full1, full2; // the next leaf / triangle pass scans every vertex
+};
+
+__device__ __forceinline__ void cands_reset(Cands &c, bool full)
 {
-    const uint64_t m = __ballot(t != kNone);
-    if (m == 0ull) return nc;  // the common case: a wave-uniform exit
-    if (t != kNone) {
-        const uint32_t pos = nc + lanes_below(m);
+    c.n1 = 0;
+    c.n2 = 0;
+    c.full1 = full;
+    c.full2 = full;
+}
+
+// Lists t by its degree `old` before a decrement (lanes with another old
+// degree, or none, add nothing; every lane must call).
+template <class N>
+__device__ __forceinline__ void add_cands(const N &w, Cands &c, int old, uint32_t t)
+{
+    const uint64_t m1 = __ballot(old == 2);
+    const uint64_t m2 = __ballot(old == 3);
+    if ((m1 | m2) == 0ull) return;  // the common case: a wave-uniform exit
+    if (old == 2) {
+        const uint32_t pos = c.n1 + lanes_below(m1);
         if (pos < w.ncap) w.cand[pos] = (typename N::cand_t)t;
+    } else if (old == 3) {
+        const uint32_t pos = c.n2 + lanes_below(m2);
+        if (pos < w.ncap) w.cand3[pos] = (typename N::cand_t)t;
     }
-    return uni(nc + (uint32_t)__popcll(m));
+    c.n1 = uni(c.n1 + (uint32_t)__popcll(m1));
+    c.n2 = uni(c.n2 + (uint32_t)__popcll(m2));
 }
 
 // Live vertex t loses its live neighbour u (several lanes may hit the same t).
@@ -480,30 +504,38 @@ __device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, typename 
 // loads serves 64 edges whatever the degrees.  Each edge is decremented
 // without a prior degree read: drop() returns the old degree, and a
 // negative one (a neighbour already in the cover) is ignored.
+// One vertex u0 (wave-uniform, live) into the cover: its id and adjacency
+// bounds are scalars, so no list walk, prefix scan or owner search.
 template <class N>
-__device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
+__device__ __forceinline__ void apply_one(const N &w, uint32_t u0, Cands &c)
+{
+    if (lane_id() == 0) w.remove(u0);
+    w.sync();
+    const uint32_t b0 = uni(w.off(u0)), e0 = uni(w.off(u0 + 1));
+#pragma unroll 1
+    for (uint32_t x0 = b0; x0 < e0; x0 += kWave) {
+        const uint32_t x = x0 + lane_id();
+        uint32_t t = kNone;
+        int old = -1;
+        if (x < e0) {
+            t = w.dst[x];
+            old = w.drop(t, u0);
+        }
+        add_cands(w, c, old, t);
+    }
+    w.sync();
+}
+
+template <class N>
+__device__ void apply_list(const N &w, uint32_t L, Cands &c)
 {
     L = uni(L);
     w.sync();
     if (L == 1u) {
-        // one listed vertex (most leaf passes): its id and adjacency bounds
-        // are wave-uniform, so no list walk, prefix scan or owner search
+        // one listed vertex (most passes of the high-degree rule)
         const uint32_t u0 = uni(w.list[0]);
-        if (lane_id() == 0) w.remove(u0);
-        w.sync();
-        const uint32_t b0 = uni(w.off(u0)), e0 = uni(w.off(u0 + 1));
-#pragma unroll 1
-        for (uint32_t x0 = b0; x0 < e0; x0 += kWave) {
-            const uint32_t x = x0 + lane_id();
-            uint32_t leaf = kNone;
-            if (x < e0) {
-                const uint32_t t = w.dst[x];
-                if (w.drop(t, u0) == 2) leaf = t;
-            }
-            nc = add_leaf_cands(w, nc, leaf);
-        }
         if (lane_id() == 0) w.mark[u0 >> 5] = 0u;
-        w.sync();
+        apply_one(w, u0, c);
         return;
     }
     for (uint32_t i = lane_id(); i < L; i += kWave) w.remove(w.list[i]);
@@ -528,12 +560,13 @@ __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
 #pragma unroll 1
             for (int x0 = 0; x0 < total; x0 += kWave) {
                 const int x = x0 + lane;
-                uint32_t leaf = kNone;
+                uint32_t t = kNone;
+                int old = -1;
                 if (x < total) {
-                    const uint32_t t = w.dst[b0 + (uint32_t)x];
-                    if (w.drop(t, u0) == 2) leaf = t;
+                    t = w.dst[b0 + (uint32_t)x];
+                    old = w.drop(t, u0);
                 }
-                nc = add_leaf_cands(w, nc, leaf);
+                add_cands(w, c, old, t);
             }
             continue;
         }
@@ -550,12 +583,13 @@ __device__ void apply_list(const N &w, uint32_t L, uint32_t &nc)
             const int own = wave_incl_max(slot[lane]);
             const int ob = __shfl(b, own), ox = __shfl(excl, own), ou = __shfl(u, own);
             const int x = x0 + lane;
-            uint32_t leaf = kNone;
+            uint32_t t = kNone;
+            int old = -1;
             if (x < total) {
-                const uint32_t t = w.dst[(uint32_t)(ob + (x - ox))];
-                if (w.drop(t, (uint32_t)ou) == 2) leaf = t;
+                t = w.dst[(uint32_t)(ob + (x - ox))];
+                old = w.drop(t, (uint32_t)ou);
             }
-            nc = add_leaf_cands(w, nc, leaf);
+            add_cands(w, c, old, t);
             w.sync();
         }
     }
@@ -596,11 +630,11 @@ __device__ __forceinline__ void lanes_add(Lanes &st, int dv, uint32_t v)
 // mark its live neighbour u when deg(u) > 1, or deg(u) == 1 and u < v (so an
 // isolated edge loses exactly one end).  Repeat until nothing is marked.
 // The reference scans every vertex in every pass.  Here a pass looks at the
-// candidate list only: a node enters the rules with no degree-1 vertex
+// leaf candidates only: a node enters the rules with no degree-1 vertex
 // (its parent ended at the rules' fixpoint) except at a root, and every later
 // degree-1 vertex got there by a decrement that listed it.  So a pass over
 // the candidates marks exactly what a full scan would.  Full scans are kept
-// for roots (full = true) and after a pass whose marks hit the list capacity
+// for roots (c.full1) and after a pass whose marks hit the list capacity
 // (the unmarked rest is found by the next full pass, as in the reference).
 template <class N>
 __device__ __forceinline__ uint32_t leaf_mark(const N &w, uint32_t v, uint32_t n)
@@ -615,29 +649,51 @@ __device__ __forceinline__ uint32_t leaf_mark(const N &w, uint32_t v, uint32_t n
 }
 
 template <class N>
-__device__ uint32_t rule_leaf(const N &w, uint32_t &nc, bool &full)
+__device__ uint32_t rule_leaf(const N &w, Cands &c)
 {
     uint32_t total = 0;
     for (;;) {
         uint32_t L = 0;
-        if (full || nc > w.ncap) {
+        if (c.full1 || c.n1 > w.ncap) {
             for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave)
                 L = append(w, L, leaf_mark(w, base + lane_id(), nv), true);
         } else {
-            const uint32_t nv = uni(w.n);
-            for (uint32_t i0 = 0; i0 < nc; i0 += kWave) {
-                const uint32_t i = i0 + lane_id();
-                L = append(w, L, leaf_mark(w, i < nc ? (uint32_t)w.cand[i] : kNone, nv), true);
+            const uint32_t nv = uni(w.n), nc = c.n1;
+            if (nc <= (uint32_t)kWave) {
+                // one chunk of candidates (the common case): a pass that marks
+                // one vertex, from one or more leaves, applies it directly
+                const uint32_t i = lane_id();
+                const uint32_t u = leaf_mark(w, i < nc ? (uint32_t)w.cand[i] : kNone, nv);
+                const uint64_t m = __ballot(u != kNone);
+                if (m == 0ull) {
+                    c.n1 = 0;
+                    c.full1 = false;
+                    break;
+                }
+                const uint32_t u0 = __builtin_amdgcn_readlane(u, (int)__builtin_ctzll(m));
+                if (__ballot(u != kNone && u != u0) == 0ull) {
+                    c.n1 = 0;  // consumed; the apply lists the next candidates
+                    apply_one(w, u0, c);
+                    total += 1;
+                    c.full1 = false;
+                    continue;
+                }
+                L = append(w, L, u, true);
+            } else {
+                for (uint32_t i0 = 0; i0 < nc; i0 += kWave) {
+                    const uint32_t i = i0 + lane_id();
+                    L = append(w, L, leaf_mark(w, i < nc ? (uint32_t)w.cand[i] : kNone, nv), true);
+                }
             }
         }
-        nc = 0;  // consumed; the apply below lists the next candidates
+        c.n1 = 0;  // consumed; the apply below lists the next candidates
         if (L == 0) {
-            full = false;
+            c.full1 = false;
             break;
         }
-        apply_list(w, L, nc);
+        apply_list(w, L, c);
         total += L;
-        full = L >= w.lcap;
+        c.full1 = L >= w.lcap;
     }
     return total;
 }
@@ -662,7 +718,7 @@ __device__ __forceinline__ bool adj_has(const uint32_t *dst, uint32_t lo, uint32
 // vertex whose two live neighbours are adjacent puts both neighbours in the
 // cover.  Skipped when a neighbour has degree 1, or degree 2 with a lower
 // index, as in the reference.  All triangles of a pass are applied together.
-// One vertex of a triangle scan: v of degree dv with sketch sum q.  Sets
+// One vertex of a triangle pass: v of degree dv with sketch sum q.  Sets
 // c1, c2 to the two neighbours to cover, or settles v.
 template <class N>
 __device__ __forceinline__ void triangle_at(const N &w, uint32_t v, int dv, typename N::s2_t q, uint32_t &c1,
@@ -684,76 +740,90 @@ __device__ __forceinline__ void triangle_at(const N &w, uint32_t v, int dv, type
     }
 }
 
+// The triangle rule over its candidates: an unexamined degree-2 vertex is
+// either a root's (c.full2: every vertex is looked at) or got there by a
+// 3 -> 2 decrement that listed it, and examined ones carry the settled bit
+// until their degree changes (kSettled).  So a pass over the candidates
+// finds every triangle a full scan would.  A pass whose marks hit the list
+// capacity (the unmarked rest stays unexamined) makes the next pass full.
 template <class N>
-__device__ uint32_t rule_triangle(const N &w, Lanes &st, uint32_t &nc)
+__device__ uint32_t rule_triangle(const N &w, Cands &c)
 {
     uint32_t total = 0;
     for (;;) {
         uint32_t L = 0;
-        lanes_reset(st);
-        // two vertices per lane: (v0, v0 + 1), so a lane's vertices still
-        // come in increasing order
-        for (uint32_t base = 0, nv = uni(w.n); base < nv; base += 2u * kWave) {
-            const uint32_t v0 = base + 2u * lane_id();
-            int d0, d1;
-            typename N::s2_t q0, q1;
-            w.pair(v0, nv, d0, d1);
-            w.pair_s2(v0, nv, q0, q1);
-            lanes_add(st, d0, v0);
-            lanes_add(st, d1, v0 + 1u);
-            uint32_t a1 = kNone, a2 = kNone, b1 = kNone, b2 = kNone;
-            triangle_at(w, v0, d0, q0, a1, a2);
-            triangle_at(w, v0 + 1u, d1, q1, b1, b2);
-            if (__ballot((a1 != kNone) | (b1 != kNone)) != 0ull) {
-                L = append(w, L, a1, true);
-                L = append(w, L, a2, true);
-                L = append(w, L, b1, true);
-                L = append(w, L, b2, true);
+        const uint32_t nv = uni(w.n);
+        if (c.full2 || c.n2 > w.ncap) {
+            // two vertices per lane: (v0, v0 + 1)
+            for (uint32_t base = 0; base < nv; base += 2u * kWave) {
+                const uint32_t v0 = base + 2u * lane_id();
+                int d0, d1;
+                typename N::s2_t q0, q1;
+                w.pair(v0, nv, d0, d1);
+                w.pair_s2(v0, nv, q0, q1);
+                uint32_t a1 = kNone, a2 = kNone, b1 = kNone, b2 = kNone;
+                triangle_at(w, v0, d0, q0, a1, a2);
+                triangle_at(w, v0 + 1u, d1, q1, b1, b2);
+                if (__ballot((a1 != kNone) | (b1 != kNone)) != 0ull) {
+                    L = append(w, L, a1, true);
+                    L = append(w, L, a2, true);
+                    L = append(w, L, b1, true);
+                    L = append(w, L, b2, true);
+                }
+            }
+        } else {
+            const uint32_t nc = c.n2;
+            for (uint32_t i0 = 0; i0 < nc; i0 += kWave) {
+                const uint32_t i = i0 + lane_id();
+                const uint32_t v = i < nc ? (uint32_t)w.cand3[i] : kNone;
+                const uint32_t vi = v < nv ? v : 0u;
+                const int dv = v < nv ? w.deg(vi) : -1;
+                const typename N::s2_t q = w.s2[vi];
+                uint32_t a1 = kNone, a2 = kNone;
+                triangle_at(w, vi, dv, q, a1, a2);
+                if (__ballot(a1 != kNone) != 0ull) {
+                    L = append(w, L, a1, true);
+                    L = append(w, L, a2, true);
+                }
             }
         }
+        c.n2 = 0;  // consumed; the apply below lists the next candidates
+        c.full2 = false;
         if (L == 0) break;
-        apply_list(w, L, nc);
+        c.full2 = L >= w.lcap;
+        apply_list(w, L, c);
         total += L;
     }
     return total;
 }
 
-// High-degree rule (src/helperFunctions.cuh:299-358): v goes into the cover
-// when deg(v) + d + (deleted so far in this call) >= T.  A pass also keeps
-// the largest degree it did not mark; degrees only fall, so when that value
-// cannot reach T with the new count, the next pass would mark nothing and is
-// not run.  st then describes the degrees before the last apply: the caller
-// runs the triangle scan again after any high-degree deletion (nh > 0), which
-// refreshes it.
+// One scan of every vertex, two per lane, for the high-degree rule
+// (src/helperFunctions.cuh:299-358: v goes into the cover when
+// deg(v) + d >= T, all marks of a pass applied together) and the node's
+// statistics in the same reads: live edge count, maximum degree and its
+// lowest index (findNumOfEdges / findMaxDegree,
+// src/helperFunctions.cuh:450-504).  Returns the number of marks (listed in
+// w.list, at most lcap; the rest are found by the next scan).  When nothing
+// is marked, st describes the node exactly.
 template <class N>
-__device__ uint32_t rule_high(const N &w, uint32_t d, uint32_t T, Lanes &st, uint32_t &nc)
+__device__ uint32_t scan_high(const N &w, uint32_t d, uint32_t T, Lanes &st)
 {
-    uint32_t total = 0;
-    for (;;) {
-        uint32_t L = 0;
-        int rest = 0;  // largest degree left unmarked by this pass (per lane)
-        lanes_reset(st);
-        for (uint32_t base = 0, nv = uni(w.n); base < nv; base += 2u * kWave) {
-            const uint32_t v0 = base + 2u * lane_id();
-            int d0, d1;
-            w.pair(v0, nv, d0, d1);  // -1 past n: adds nothing below
-            lanes_add(st, d0, v0);
-            lanes_add(st, d1, v0 + 1u);
-            const bool h0 = (d0 > 0) & ((uint32_t)d0 + d + total >= T);
-            const bool h1 = (d1 > 0) & ((uint32_t)d1 + d + total >= T);
-            rest = h0 ? rest : max(rest, d0);
-            rest = h1 ? rest : max(rest, d1);
-            if (__ballot(h0 | h1) != 0ull) {
-                L = append(w, L, h0 ? v0 : kNone, false);
-                L = append(w, L, h1 ? v0 + 1u : kNone, false);
-            }
+    uint32_t L = 0;
+    lanes_reset(st);
+    for (uint32_t base = 0, nv = uni(w.n); base < nv; base += 2u * kWave) {
+        const uint32_t v0 = base + 2u * lane_id();
+        int d0, d1;
+        w.pair(v0, nv, d0, d1);  // -1 past n: adds nothing below
+        lanes_add(st, d0, v0);
+        lanes_add(st, d1, v0 + 1u);
+        const bool h0 = (d0 > 0) & ((uint32_t)d0 + d >= T);
+        const bool h1 = (d1 > 0) & ((uint32_t)d1 + d >= T);
+        if (__ballot(h0 | h1) != 0ull) {
+            L = append(w, L, h0 ? v0 : kNone, false);
+            L = append(w, L, h1 ? v0 + 1u : kNone, false);
         }
-        if (L == 0) break;
-        apply_list(w, L, nc);
-        total += L;
-        if ((uint32_t)wave_max(rest) + d + total < T) break;
     }
-    return total;
+    return L;
 }
 
 struct Summary {
@@ -762,39 +832,34 @@ struct Summary {
     uint32_t maxv;   // lowest index of maximum degree
 };
 
-// Rule fixpoint in the order of src/GlobalWorkList.cuh:107-130.  Returns the
-// new number of vertices in the cover and, in *sum, the node's edge count and
-// max-degree vertex.  nc / full carry the leaf candidates (see rule_leaf).
-// The triangle rule scans every vertex whenever it runs (always in the
-// first round), and its last scan marks nothing, so the statistics gathered
-// by the latest triangle or high-degree scan are exact for the current
-// degrees: the high-degree rule is not scanned when max degree + d < T (its
-// first scan could not mark anything), and the summary needs no scan of its
-// own.  The rules that run, their order and their results are unchanged.
+// Rule fixpoint of one node.  The reference applies leaf, triangle and
+// high-degree passes in a fixed round (src/GlobalWorkList.cuh:107-130) with
+// every pass a scan of all vertices.  Here the leaf and triangle rules run
+// from their candidate lists until neither marks anything, then one scan of
+// every vertex applies the high-degree rule and gathers the node's
+// statistics; after a high-degree deletion the candidate-driven rules run
+// again.  The node leaves when that scan marks nothing: every rule is then at
+// its fixpoint, as at the end of the reference's round, and each deletion is
+// one of the three rules applied to the current graph, so the search stays
+// exact (the tree can differ, as the reference's GPU and CPU trees already
+// do).  Returns the new cover size and, in *sum, the node's edge count and
+// max-degree vertex.
 template <class N>
-__device__ uint32_t reduce_node(const N &w, uint32_t d, uint32_t T, Summary *sum, uint32_t &nc, bool &full)
+__device__ uint32_t reduce_node(const N &w, uint32_t d, uint32_t T, Summary *sum, Cands &c)
 {
-    uint32_t it = 0, nl, nt = 0, nh = 0;
     Lanes st;
-    do {
-        nl = rule_leaf(w, nc, full);
-        d += nl;
-        nt = 0;
-        if (it == 0 || nl > 0 || nh > 0) {
-            nt = rule_triangle(w, st, nc);
+    for (;;) {
+        for (;;) {
+            d += rule_leaf(w, c);
+            const uint32_t nt = rule_triangle(w, c);
             d += nt;
+            if (nt == 0) break;  // the leaf rule was at its fixpoint before this triangle pass
         }
-        nh = 0;
-        if (it == 0 || nl > 0 || nt > 0) {
-            // the triangle rule ran in this round: st is exact
-            const int md = wave_max(st.md);
-            if ((uint32_t)md + d >= T) {
-                nh = rule_high(w, d, T, st, nc);
-                d += nh;
-            }
-        }
-        ++it;
-    } while (nt > 0 || nh > 0);
+        const uint32_t L = scan_high(w, d, T, st);
+        if (L == 0) break;
+        apply_list(w, L, c);
+        d += L;
+    }
     const int md = wave_max(st.md);
     sum->edges = (uint32_t)wave_sum(st.s) / 2u;
     sum->maxdeg = md;
@@ -805,7 +870,7 @@ __device__ uint32_t reduce_node(const N &w, uint32_t d, uint32_t T, Summary *sum
 // Left child in place: v into the cover (deleteMaxDegreeVertex,
 // src/helperFunctions.cuh:200-215).
 template <class N>
-__device__ void take_vertex(const N &w, uint32_t v, uint32_t &nc)
+__device__ void take_vertex(const N &w, uint32_t v, Cands &c)
 {
     w.sync();
     if (lane_id() == 0) w.remove(v);
@@ -813,17 +878,15 @@ __device__ void take_vertex(const N &w, uint32_t v, uint32_t &nc)
     const uint32_t b = w.off(v), e = w.off(v + 1);
     for (uint32_t x0 = b; x0 < e; x0 += kWave) {
         const uint32_t x = x0 + lane_id();
-        uint32_t leaf = kNone;
+        uint32_t t = kNone;
+        int dt = -1;
         if (x < e) {
-            const uint32_t t = w.dst[x];
-            const int dt = w.deg(t);
-            if (dt >= 0) {
-                // the neighbours of one vertex are distinct: plain updates
-                w.drop_plain(t, v, dt);
-                if (dt == 2) leaf = t;
-            }
+            t = w.dst[x];
+            dt = w.deg(t);
+            // the neighbours of one vertex are distinct: plain updates
+            if (dt >= 0) w.drop_plain(t, v, dt);
         }
-        nc = add_leaf_cands(w, nc, leaf);
+        add_cands(w, c, dt, t);
     }
     w.sync();
 }
@@ -833,7 +896,7 @@ __device__ void take_vertex(const N &w, uint32_t v, uint32_t &nc)
 // rounds of at most lcap vertices.  Returns how many vertices were added
 // (deg(v) before the call).
 template <class N>
-__device__ uint32_t take_neighbours(const N &w, uint32_t v, uint32_t &nc)
+__device__ uint32_t take_neighbours(const N &w, uint32_t v, Cands &c)
 {
     uint32_t total = 0;
     const uint32_t b = w.off(v), e = w.off(v + 1);
@@ -849,7 +912,7 @@ __device__ uint32_t take_neighbours(const N &w, uint32_t v, uint32_t &nc)
             L = append(w, L, cand, false);
         }
         if (L == 0) break;
-        apply_list(w, L, nc);
+        apply_list(w, L, c);
         total += L;
         if (L < w.lcap) break;  // no entry was dropped: every live neighbour is done
     }
@@ -885,9 +948,18 @@ __device__ void init_root(const N &w, const int32_t *deg0, uint32_t npad)
 // A slot is sized for the largest graph of the launch; only the current
 // graph's vertices travel (rounded up to whole 16-byte vectors).  The packed
 // layout's two arrays are copied as one run of 16-byte vectors over both
-// (the second array starts npad * 4 bytes in, npad a multiple of 32): one
-// vector in flight per lane keeps the copy within the kernel's register
-// budget.
+// (the second array starts npad * 4 bytes in, npad a multiple of 32).  Each
+// lane loads up to four vectors before it stores any, so a copy from HBM
+// (a pop or a steal) waits for one round trip per 256 vectors (4 KB)
+// instead of one per 64.
+#ifndef VC_LDS_DIRECT
+#define VC_LDS_DIRECT 1
+#endif
+#ifndef VC_COPY_VECS
+#define VC_COPY_VECS 2
+#endif
+constexpr uint32_t kCopyVecs = VC_COPY_VECS;  // 16-byte vectors in flight per lane
+
 template <class N>
 __device__ __forceinline__ void copy_state(char *__restrict__ dst, const char *__restrict__ src, uint32_t npad,
                                            uint32_t cnt)
@@ -898,9 +970,19 @@ __device__ __forceinline__ void copy_state(char *__restrict__ dst, const char *_
     const uint32_t o1 = npad >> 2;       // second array, at npad * 4 bytes
     if constexpr (N::kPacked) {
         // vector k < q is entry k of the first array, k >= q entry k - q of the second
-        for (uint32_t k = lane_id(); k < 2u * q; k += kWave) {
-            const uint32_t i = k < q ? k : o1 + (k - q);
-            d4[i] = s4[i];
+        const uint32_t total = 2u * q;
+        for (uint32_t k0 = lane_id(); k0 < total; k0 += kCopyVecs * kWave) {
+            int4 x[kCopyVecs];
+#pragma unroll
+            for (uint32_t j = 0; j < kCopyVecs; ++j) {
+                const uint32_t k = k0 + j * kWave;
+                if (k < total) x[j] = s4[k < q ? k : o1 + (k - q)];
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kCopyVecs; ++j) {
+                const uint32_t k = k0 + j * kWave;
+                if (k < total) d4[k < q ? k : o1 + (k - q)] = x[j];
+            }
         }
     } else {
         const uint32_t o2 = npad >> 1;   // 8-byte s2 entries, at npad * 8 bytes
@@ -912,6 +994,32 @@ __device__ __forceinline__ void copy_state(char *__restrict__ dst, const char *_
             d4[o2 + 2u * i + 1u] = z1;
         }
     }
+}
+
+// A stack or worklist slot (HBM) into LDS node storage, packed layout:
+// every lane's 16-byte vector goes straight from global memory into LDS
+// (global_load_lds_dwordx4: lane i of a load writes 16 bytes at the
+// wave-uniform LDS address + 16 i), with no registers in between, so every
+// load of the copy is in flight at once and the copy waits for one round
+// trip.  Lanes past the copied vectors are masked off and write nothing.
+template <class N>
+__device__ __forceinline__ void copy_state_to_lds(char *dst, const char *src, uint32_t npad, uint32_t cnt)
+{
+    static_assert(N::kPacked && !N::kHbm, "LDS node storage, packed layout");
+    const uint32_t q = (cnt + 3u) >> 2;  // 16-byte vectors per array of 4-byte entries
+    const uint32_t abytes = npad * 4u;   // second array at npad * 4 bytes
+#pragma unroll
+    for (uint32_t arr = 0; arr < 2u; ++arr) {
+        const char *s = src + arr * abytes;
+        char *d = dst + arr * abytes;
+        for (uint32_t k0 = 0; k0 < q; k0 += kWave) {
+            const uint32_t k = k0 + lane_id();
+            if (k < q)
+                __builtin_amdgcn_global_load_lds(reinterpret_cast<const int4 *>(s) + k,
+                                                 reinterpret_cast<int4 *>(d) + k0, 16, 0, 0);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0): every vector has landed in LDS
 }
 
 // HBM bytes one copy_state of cnt vertices reads or writes on the slot side
@@ -1007,8 +1115,12 @@ __device__ bool try_dequeue(const SolveArgs &a, const N &w, uint32_t &g, uint32_
     g = uni(meta.x);
     d = uni(meta.y);
     v = uni(meta.z);
-    copy_state<N>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, a.npad,
-                  uni(a.gdesc[g].n));
+    if constexpr (N::kPacked && !N::kHbm && VC_LDS_DIRECT)
+        copy_state_to_lds<N>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, a.npad,
+                             uni(a.gdesc[g].n));
+    else
+        copy_state<N>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, a.npad,
+                      uni(a.gdesc[g].n));
     w.sync();
     fence_rel();
     if (lane_id() == 0) {
@@ -1369,9 +1481,10 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
     w.mark = reinterpret_cast<uint32_t *>(w.list + a.lcap);  // lcap is a multiple of 4
     w.xs = reinterpret_cast<int *>(w.mark + a.mwords);
     w.cand = reinterpret_cast<typename NodeT::cand_t *>(w.xs + kWave);
-    w.ncap = a.npad;
+    w.cand3 = w.cand + a.ncap;
+    w.ncap = a.ncap;
     w.src = nullptr;
-    w.off16 = OFF16 ? reinterpret_cast<uint16_t *>(w.cand + a.npad) : nullptr;
+    w.off16 = OFF16 ? reinterpret_cast<uint16_t *>(w.cand3 + a.ncap) : nullptr;
     w.dst = nullptr;
     w.n = 0;
     w.lcap = a.lcap;
@@ -1396,9 +1509,8 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
     bool have = false;
     uint32_t d = 0;
     uint32_t path = 0, descend = 0;
-    uint32_t nc = 0;    // leaf candidates listed since the last leaf pass
+    Cands c;            // rule candidates listed since the last passes
     uint32_t off16_g = kNone;  // graph whose offsets the LDS copy holds
-    bool full = true;   // the next leaf pass must scan every vertex
     uint64_t pending = 0, total_nodes = 0, idle_ticks = 0, donations = 0, steals = 0;
     // slot copies of the current graph's node state since the last flush:
     // stack pushes and pops, steals (ring slot in) and donations (stack slot
@@ -1416,15 +1528,17 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 // right child
                 const uint32_t ts = (uint32_t)top % a.depth;
                 const uint2 m = stk_meta[ts];  // in flight during the copy
-                copy_state<NodeT>(w.base, stk + (size_t)ts * slot_bytes, a.npad, w.n);
+                if constexpr (NodeT::kPacked && !HBM && VC_LDS_DIRECT)
+                    copy_state_to_lds<NodeT>(w.base, stk + (size_t)ts * slot_bytes, a.npad, w.n);
+                else
+                    copy_state<NodeT>(w.base, stk + (size_t)ts * slot_bytes, a.npad, w.n);
                 ++copies;
                 w.sync();
                 d = uni(m.x);
                 uint32_t v = uni(m.y);
                 --top;
-                nc = 0;
-                full = false;
-                d += take_neighbours(w, v, nc);
+                cands_reset(c, false);
+                d += take_neighbours(w, v, c);
                 have = true;
             } else {
                 if (g != kNone) {
@@ -1455,15 +1569,14 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                     off16_g = g;
                 }
                 w.dst = a.gbuf + gd.off_dst;
-                nc = 0;
                 if (q.kind == 1) {
                     d = q.d;
-                    full = q.v == kNone;
-                    if (q.v != kNone) d += take_neighbours(w, q.v, nc);
+                    cands_reset(c, q.v == kNone);
+                    if (q.v != kNone) d += take_neighbours(w, q.v, c);
                     descend = 0;
                 } else {
                     init_root(w, reinterpret_cast<const int32_t *>(a.gbuf + gd.off_deg), a.npad);
-                    full = true;
+                    cands_reset(c, true);
                     d = 0;
                     if (a.version == 1) {
                         descend = 0;
@@ -1504,7 +1617,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         const bool mvc = gd.instance == 0;
         uint32_t T = mvc ? uni(best_now) : gd.k + 1u;
         Summary s;
-        d = reduce_node(w, d, T, &s, nc, full);
+        d = reduce_node(w, d, T, &s, c);
 
         // the live incumbent (another worker may have lowered it while the
         // rules ran with T); the bound uses both (bound_prunes)
@@ -1526,10 +1639,10 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
             uint32_t bit = (path >> (a.start_depth - descend)) & 1u;
             --descend;
             if (bit == 0u) {
-                take_vertex(w, v, nc);
+                take_vertex(w, v, c);
                 d += 1;
             } else {
-                d += take_neighbours(w, v, nc);
+                d += take_neighbours(w, v, c);
             }
             continue;
         }
@@ -1591,7 +1704,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 }
             }
         }
-        take_vertex(w, v, nc);
+        take_vertex(w, v, c);
         d += 1;
     }
     if (lane_id() == 0) {
@@ -1979,17 +2092,19 @@ const void *kernel_of(int variant, bool serve, bool off16 = false)
 
 struct Plan {
     int variant;
-    uint32_t npad, lcap, mwords, worker_bytes, deg_bytes, slot_bytes, wpb, block_dim, blocks_per_cu, num_blocks,
-        workers;
+    uint32_t npad, lcap, ncap, mwords, worker_bytes, deg_bytes, slot_bytes, wpb, block_dim, blocks_per_cu,
+        num_blocks, workers;
     uint32_t depth, qsize, qthreshold, roots, units;
     uint32_t off16;  // workers keep their graph's srcPtr in LDS as uint16
     size_t smem;
     size_t off_glob, off_stk, off_meta, off_qdeg, off_qmeta, off_tickets, off_wnodes, off_hbm, total;
 };
 
-// Per-worker node storage for a degree type: deg [npad] | s1 [npad] | s2 [npad] | list [lcap] |
-// mark [mwords] | xs [64] | cand [npad] | (off16) srcPtr [npad + 2] uint16.  A stack / worklist
-// slot is deg | s1 | s2.
+constexpr uint32_t kCandCap = 256;  // entries per rule-candidate list
+
+// Per-worker node storage for a degree type: node state [npad] | list [lcap] | mark [mwords] |
+// xs [64] | cand [ncap] | cand3 [ncap] | (off16) srcPtr [npad + 2] uint16.  A stack / worklist
+// slot is the node state.
 void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = false)
 {
     pl.deg_bytes = deg_bytes;
@@ -1997,9 +2112,12 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = fals
     pl.slot_bytes = pl.npad * (deg_bytes == 2u ? vcg::node_bytes_per_vertex<int16_t>()
                                                : vcg::node_bytes_per_vertex<int32_t>());
     pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), 1024u);
+    // candidate lists: a longer run of candidates between two passes of a
+    // rule makes that pass scan every vertex instead
+    pl.ncap = std::min<uint32_t>(round_up(nmax, 4u), kCandCap);
     const uint32_t id_bytes = deg_bytes == 2u ? 2u : 4u;  // mark and candidate list entries
     pl.mwords = round_up((nmax + 31u) / 32u, 4u);
-    const uint32_t cand_bytes = pl.npad * id_bytes;
+    const uint32_t cand_bytes = 2u * pl.ncap * id_bytes;
     pl.worker_bytes =
         round_up(pl.slot_bytes + id_bytes * pl.lcap + 4u * pl.mwords + 4u * vcg::kWave + cand_bytes +
                      (off16 ? 2u * (pl.npad + 2u) : 0u),
@@ -2161,6 +2279,7 @@ void fill_args(SolveArgs &a, const Plan &pl, char *base, uint32_t version, uint3
     a.wstats = reinterpret_cast<vcg::WorkerStats *>(base + pl.off_wnodes);
     a.npad = pl.npad;
     a.lcap = pl.lcap;
+    a.ncap = pl.ncap;
     a.mwords = pl.mwords;
     a.worker_bytes = pl.worker_bytes;
     a.depth = pl.depth;
