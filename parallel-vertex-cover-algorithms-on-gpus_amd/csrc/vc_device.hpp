@@ -22,7 +22,11 @@ struct GDesc {
     uint32_t U;          // upper bound (valid cover size)
     uint32_t seq;        // serving: request sequence number echoed in the completion
     uint64_t max_nodes;  // 0 = unlimited
-    uint32_t pad1[4];
+    // first word of the graph's adjacency bitmap (bit n1 * n + n2 set when
+    // n1 and n2 are adjacent), 0 = none (the triangle rule then searches the
+    // sorted adjacency list)
+    uint32_t off_bits;
+    uint32_t pad1[3];
 };
 static_assert(sizeof(GDesc) == 64, "GDesc is one 64-byte line");
 // the serving loader reads these fields by 32-bit word index

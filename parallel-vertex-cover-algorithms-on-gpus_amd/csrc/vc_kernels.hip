@@ -247,6 +247,7 @@ struct Node {
     const uint32_t *src; // graph srcPtr (HBM)
     uint16_t *off16;     // [n+1] the same offsets in LDS, or nullptr (see off())
     const uint32_t *dst; // graph dst (HBM)
+    const uint32_t *bits; // graph adjacency bitmap (HBM), or nullptr
     uint32_t n;
     uint32_t lcap;
 
@@ -731,7 +732,14 @@ __device__ __forceinline__ void triangle_at(const N &w, uint32_t v, int dv, type
         // lower index (src/helperFunctions.cuh:379-385)
         const int d1 = w.deg(n1), d2 = w.deg(n2);
         const bool skip = (d1 == 1) | ((d1 == 2) & (n1 < v)) | (d2 == 1) | ((d2 == 2) & (n2 < v));
-        if (!skip && adj_has(w.dst, w.off(n2), w.off(n2 + 1), n1)) {
+        bool adj;
+        if (w.bits) {
+            const uint32_t bit = n1 * w.n + n2;
+            adj = (w.bits[bit >> 5] >> (bit & 31u)) & 1u;
+        } else {
+            adj = adj_has(w.dst, w.off(n2), w.off(n2 + 1), n1);
+        }
+        if (!skip && adj) {
             c1 = n1;
             c2 = n2;
         } else {
@@ -1433,6 +1441,7 @@ __device__ __forceinline__ GDesc load_desc(const GDesc *p)
     d.off_dst = uni(d.off_dst);
     d.off_deg = uni(d.off_deg);
     d.off_cover = uni(d.off_cover);
+    d.off_bits = uni(d.off_bits);
     d.instance = uni(d.instance);
     d.k = uni(d.k);
     d.U = uni(d.U);
@@ -1486,6 +1495,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
     w.src = nullptr;
     w.off16 = OFF16 ? reinterpret_cast<uint16_t *>(w.cand3 + a.ncap) : nullptr;
     w.dst = nullptr;
+    w.bits = nullptr;
     w.n = 0;
     w.lcap = a.lcap;
     for (uint32_t i = lane_id(); i < a.mwords; i += kWave) w.mark[i] = 0u;
@@ -1569,6 +1579,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                     off16_g = g;
                 }
                 w.dst = a.gbuf + gd.off_dst;
+                w.bits = gd.off_bits ? a.gbuf + gd.off_bits : nullptr;
                 if (q.kind == 1) {
                     d = q.d;
                     cands_reset(c, q.v == kNone);
@@ -2101,6 +2112,10 @@ struct Plan {
 };
 
 constexpr uint32_t kCandCap = 256;  // entries per rule-candidate list
+// graphs up to this many vertices get an adjacency bitmap (128 KB at most),
+// while the graph set stays below 2^31 words
+constexpr uint32_t kAdjBitsMaxN = 1024;
+constexpr uint64_t kAdjBitsMaxWords = 1ull << 31;
 
 // Per-worker node storage for a degree type: node state [npad] | list [lcap] | mark [mwords] |
 // xs [64] | cand [ncap] | cand3 [ncap] | (off16) srcPtr [npad + 2] uint16.  A stack / worklist
@@ -2418,6 +2433,14 @@ vc_status vc_graphs_upload(vc_ctx *ctx, int count, const vc_csr *graphs, vc_grap
         d.off_deg = (uint32_t)words;
         words += g.vertexNum;
         words = (words + 3) / 4 * 4;
+        // adjacency bitmap for the triangle rule's pair test (one load
+        // instead of a binary search) while it stays small
+        const uint64_t bit_words = ((uint64_t)g.vertexNum * g.vertexNum + 31u) / 32u;
+        d.off_bits = 0;
+        if (g.vertexNum <= kAdjBitsMaxN && words + bit_words < kAdjBitsMaxWords) {
+            d.off_bits = (uint32_t)words;
+            words += (bit_words + 3) / 4 * 4;
+        }
         d.off_cover = cover_words;
         cover_words += round_up((g.vertexNum + 31u) / 32u, 2u);
         s->nmax = std::max(s->nmax, g.vertexNum);
@@ -2443,6 +2466,12 @@ vc_status vc_graphs_upload(vc_ctx *ctx, int count, const vc_csr *graphs, vc_grap
             host[d.off_src] = 0u;
         }
         if (g.edgeNum) std::memcpy(&host[d.off_dst], g.dst, sizeof(uint32_t) * 2ull * g.edgeNum);
+        if (d.off_bits)
+            for (uint32_t v = 0; v < g.vertexNum; ++v)
+                for (uint32_t x = g.srcPtr[v]; x < g.srcPtr[v + 1]; ++x) {
+                    const uint64_t bit = (uint64_t)v * g.vertexNum + g.dst[x];
+                    host[d.off_bits + (bit >> 5)] |= 1u << (bit & 31u);
+                }
     }
     bool ok = hip_ok(hipMalloc(&s->d_words, host.size() * sizeof(uint32_t))) &&
               hip_ok(hipMalloc(&s->d_desc, sizeof(GDesc) * count)) &&
