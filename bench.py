@@ -398,13 +398,15 @@ def batch_leg(ctx, args, params, out):
     gset = ctx.upload(graphs)
     upload_s = time.perf_counter() - t_up
     # launch shape: the autotuner on a 512-graph sample of the same
-    # distribution (short node budget), then graph-captured dispatch
+    # distribution with the step's own node budget (shorter budgets end the
+    # sample's launches before the per-node rate that decides a full step
+    # settles), then graph-captured dispatch
     tuned = None
     if not args.no_tune:
         t0 = time.perf_counter()
         ns = min(512, len(graphs))
         sample = ctx.upload(graphs[:ns])
-        sjobs = [vcgpu.Job(vcgpu.VC_MVC, 0, u, 1 << 16) for u, _ in ubs[:ns]]
+        sjobs = [vcgpu.Job(vcgpu.VC_MVC, 0, u, args.max_nodes) for u, _ in ubs[:ns]]
         best, _ = ctx.autotune(sample, sjobs, params)
         sample.free()
         params = vcgpu.Params(version=params.version, block_dim=best.block_dim, num_blocks=args.num_blocks,

@@ -510,8 +510,9 @@ __device__ __forceinline__ bool two_neighbours(const N &w, uint32_t v, typename 
 template <class N>
 __device__ __forceinline__ void apply_one(const N &w, uint32_t u0, Cands &c)
 {
+    // u0 is not among its own neighbours: its removal and the decrements
+    // below touch different vertices and need no ordering between them
     if (lane_id() == 0) w.remove(u0);
-    w.sync();
     const uint32_t b0 = uni(w.off(u0)), e0 = uni(w.off(u0 + 1));
 #pragma unroll 1
     for (uint32_t x0 = b0; x0 < e0; x0 += kWave) {
@@ -653,6 +654,7 @@ template <class N>
 __device__ uint32_t rule_leaf(const N &w, Cands &c)
 {
     uint32_t total = 0;
+    if (!c.full1 && c.n1 == 0u) return 0;  // nothing listed: nothing to mark
     for (;;) {
         uint32_t L = 0;
         if (c.full1 || c.n1 > w.ncap) {
@@ -960,13 +962,9 @@ __device__ void init_root(const N &w, const int32_t *deg0, uint32_t npad)
 // lane loads up to four vectors before it stores any, so a copy from HBM
 // (a pop or a steal) waits for one round trip per 256 vectors (4 KB)
 // instead of one per 64.
-#ifndef VC_LDS_DIRECT
-#define VC_LDS_DIRECT 1
-#endif
-#ifndef VC_COPY_VECS
-#define VC_COPY_VECS 2
-#endif
-constexpr uint32_t kCopyVecs = VC_COPY_VECS;  // 16-byte vectors in flight per lane
+// 16-byte vectors in flight per lane in a register copy between node
+// storage and a slot (measured: 2 beat 1 by 0.8% and 4, which spills, lost 3%)
+constexpr uint32_t kCopyVecs = 2;
 
 template <class N>
 __device__ __forceinline__ void copy_state(char *__restrict__ dst, const char *__restrict__ src, uint32_t npad,
@@ -1123,7 +1121,7 @@ __device__ bool try_dequeue(const SolveArgs &a, const N &w, uint32_t &g, uint32_
     g = uni(meta.x);
     d = uni(meta.y);
     v = uni(meta.z);
-    if constexpr (N::kPacked && !N::kHbm && VC_LDS_DIRECT)
+    if constexpr (N::kPacked && !N::kHbm)
         copy_state_to_lds<N>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, a.npad,
                              uni(a.gdesc[g].n));
     else
@@ -1538,7 +1536,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 // right child
                 const uint32_t ts = (uint32_t)top % a.depth;
                 const uint2 m = stk_meta[ts];  // in flight during the copy
-                if constexpr (NodeT::kPacked && !HBM && VC_LDS_DIRECT)
+                if constexpr (NodeT::kPacked && !HBM)
                     copy_state_to_lds<NodeT>(w.base, stk + (size_t)ts * slot_bytes, a.npad, w.n);
                 else
                     copy_state<NodeT>(w.base, stk + (size_t)ts * slot_bytes, a.npad, w.n);
@@ -1875,11 +1873,12 @@ __global__ __launch_bounds__(1024) void solve_kernel(SolveArgs a)
 
 template __global__ void solve_kernel<int16_t, false, false>(SolveArgs);
 // Batch launches whose graphs all have 2m < 2^16 keep each worker's current
-// srcPtr in LDS (OFF16).  hipcc allocates 81 VGPRs for that body (5 waves
-// per SIMD); asking for 6 waves per EU gets 80 with one register spilled.
-// Measured per batched step: 3.67 s against 4.15 s at 5 waves and 3.88 s
-// for the HBM-offset kernel at 7 waves.
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void solve_kernel_off16(SolveArgs a)
+// srcPtr in LDS (OFF16).  Asking for 7 waves per SIMD caps the body at 72
+// VGPRs (about 40 spilled, mostly off the per-node path); with 64-entry
+// candidate lists a worker's LDS at n = 400 lets 28 workers share a CU.
+// Measured per batched step: 2.87 s at 7 waves against 3.16 s at 6 (80
+// VGPRs) and 3.38 s at 8 (64 VGPRs, 68 spilled).
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7))) void solve_kernel_off16(SolveArgs a)
 {
     solve_body<int16_t, false, false, true>(a);
 }
@@ -2111,7 +2110,10 @@ struct Plan {
     size_t off_glob, off_stk, off_meta, off_qdeg, off_qmeta, off_tickets, off_wnodes, off_hbm, total;
 };
 
-constexpr uint32_t kCandCap = 256;  // entries per rule-candidate list
+// entries per rule-candidate list: small enough that 28 workers' LDS fits a
+// CU at n = 400 (a longer run of candidates makes the next pass scan all)
+constexpr uint32_t kCandCap = 64;
+constexpr uint32_t kListCap = 1024;  // marks per rule pass (the rest go to the next pass)
 // graphs up to this many vertices get an adjacency bitmap (128 KB at most),
 // while the graph set stays below 2^31 words
 constexpr uint32_t kAdjBitsMaxN = 1024;
@@ -2126,7 +2128,7 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = fals
     pl.npad = round_up(nmax, 64u / deg_bytes);
     pl.slot_bytes = pl.npad * (deg_bytes == 2u ? vcg::node_bytes_per_vertex<int16_t>()
                                                : vcg::node_bytes_per_vertex<int32_t>());
-    pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), 1024u);
+    pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), kListCap);
     // candidate lists: a longer run of candidates between two passes of a
     // rule makes that pass scan every vertex instead
     pl.ncap = std::min<uint32_t>(round_up(nmax, 4u), kCandCap);
