@@ -50,7 +50,10 @@ import graphgen  # noqa: E402
 import vcgpu  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_SOURCES = [os.path.join(PKG, "csrc", f) for f in ("vc_kernels.hip", "vc_device.hpp")]
+KERNEL_SOURCES = [os.path.join(PKG, "csrc", f) for f in ("vc_kernels.hip", "vc_device.hpp", "vc_bound.hpp")]
+# vc_kernels.hip holds the device code first; the host launcher below this
+# marker does not change the kernels a profile measured
+HOST_MARKER = "host launcher and C ABI"
 BATCH_PROFILE = os.path.join(ROOT, "profiles", "solve_kernel_batch4096.md")
 
 
@@ -84,13 +87,16 @@ def pct(a, q):
     return float(np.percentile(a, q)) if len(a) else None
 
 
-def kernel_source_sha() -> str:
-    """Hash of the kernel sources (code only) a committed profile was measured on."""
+def kernel_source_sha(sources=None) -> str:
+    """Hash of the device code (comments dropped, host launcher cut off) a
+    committed profile was measured on."""
     import hashlib
     h = hashlib.sha256()
-    for f in KERNEL_SOURCES:
+    for f in sources or KERNEL_SOURCES:
         with open(f, encoding="utf-8") as fh:
-            h.update(_code_only(fh.read()).encode())
+            text = fh.read()
+        cut = text.find(HOST_MARKER)
+        h.update(_code_only(text if cut < 0 else text[:cut]).encode())
     return h.hexdigest()[:16]
 
 

@@ -2832,10 +2832,13 @@ extern "C" vc_status vc_autotune(vc_ctx *ctx, const vc_graphs *set, const vc_job
         return VC_OK;
     }
     // candidates: workgroup of 1, 2 or 4 worker waves; node storage in LDS or
-    // HBM; worklist sized for the grid or 4x larger
+    // HBM; worklist sized for the grid or 4x larger.  LDS shapes go first:
+    // where they fit they are the fast ones, and each later candidate runs
+    // under a time limit of twice the best kernel time so far, so a slow
+    // shape ends early instead of running the whole sample.
     std::vector<vc_params> cands;
-    for (uint32_t bd : {64u, 128u, 256u})
-        for (int gm : {0, 1})
+    for (int gm : {0, 1})
+        for (uint32_t bd : {64u, 128u, 256u})
             for (uint32_t qmul : {0u, 4u}) {
                 vc_params p = *base;
                 p.block_dim = bd;
@@ -2851,11 +2854,15 @@ extern "C" vc_status vc_autotune(vc_ctx *ctx, const vc_graphs *set, const vc_job
                 cands.push_back(p);
             }
     std::vector<vc_result> res(set->count);
-    float best_t = 0.f;
+    float best_t = 0.f, best_kernel_ms = 0.f;
     int best_i = -1;
     for (size_t i = 0; i < cands.size(); ++i) {
         Plan pl;
         if (make_plan(ctx, set, jobs, &cands[i], pl) != VC_OK) continue;
+        if (best_i >= 0) {
+            const uint32_t cap = (uint32_t)std::max(1000.f, 2.f * best_kernel_ms);
+            if (!cands[i].time_limit_ms || cap < cands[i].time_limit_ms) cands[i].time_limit_ms = cap;
+        }
         vc_launch_info info;
         vc_status st = vc_solve(ctx, set, jobs, &cands[i], res.data(), nullptr, &info);
         if (st != VC_OK && st != VC_ETIMEOUT) continue;
@@ -2866,6 +2873,7 @@ extern "C" vc_status vc_autotune(vc_ctx *ctx, const vc_graphs *set, const vc_job
         const float t = info.kernel_ms / (float)std::max<uint32_t>(solved, 1u);
         if (best_i < 0 || t < best_t) {
             best_t = t;
+            best_kernel_ms = info.kernel_ms;
             best_i = (int)i;
         }
     }
