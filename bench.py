@@ -275,7 +275,7 @@ def reference_speed_check(n: int = 150, p: float = 0.05, seed: int = 3):
             "agree": ref["result"] == r and ref["nodes"] == st["nodes"]}
 
 
-def single_leg(ctx, name: str, reps: int = 5, cpu_budget_s: float = 10.0):
+def single_leg(ctx, name: str, reps: int = 5, cpu_budget_s: float = 10.0, params=None):
     """One single-graph workload: GPU median of `reps` warm solves (kernel and
     wall time), CPU oracle on 1 core (median of `reps` full solves when the
     oracle's node count is small, else a bounded sample giving tree-nodes/s
@@ -285,7 +285,7 @@ def single_leg(ctx, name: str, reps: int = 5, cpu_budget_s: float = 10.0):
     g = vcgpu.csr_from_edges(n, e)
     U, _ = vcgpu.upper_bound(g, seed=0)
     gs = ctx.upload(g)
-    params = vcgpu.Params(time_limit_ms=120000)
+    params = params or vcgpu.Params(time_limit_ms=120000)
     if name == "single_ba":
         (r0,), _, _ = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), params, want_covers=False)
         if not r0.solved:
@@ -603,7 +603,7 @@ def main():
                     "config": sv, "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS,
                                                "unit": "GB/s", "frac": None, "traffic": None}})
     else:
-        sg = single_leg(ctx, args.workload, reps=max(args.steps, 1))
+        sg = single_leg(ctx, args.workload, reps=max(args.steps, 1), params=params)
         out.update({"value": len(sg["answers"]) / (sg["gpu_ms_median"] / 1000.0), "unit": "graphs/s",
                     "ms_per_step": sg["gpu_ms_median"], "config": dict(workload=args.workload, **sg),
                     "roofline": {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
