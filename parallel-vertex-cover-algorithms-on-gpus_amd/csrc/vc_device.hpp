@@ -73,8 +73,11 @@ struct Global {
     uint32_t cancel;          // cancel request (host, or the deadline)
     uint32_t timed_out;       // the deadline passed before the search finished
     uint64_t t0;              // wall clock when the first solve worker started (0 = none yet)
+    uint32_t pad5[10];
+    // on a line of its own: every finished unit adds to it, and the line
+    // above is read by every idle poll
     unsigned long long state_bytes;  // HBM bytes of node-state slot copies (stacks, worklist)
-    uint32_t pad5[8];
+    uint32_t pad7[14];
     // serving: device copy of the host's request tail / shutdown flag, and the
     // time of the last host poll (polls are rate-limited to one per interval)
     uint32_t host_tail;

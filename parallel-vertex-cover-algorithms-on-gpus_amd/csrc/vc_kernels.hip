@@ -171,10 +171,52 @@ __device__ __forceinline__ T fetch_add(T *p, T v)
 __device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 __device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
+// Worklist hand-offs without agent-scope release/acquire.  On gfx950 every
+// XCD has its own L2, so an agent-scope release writes back the XCD's whole
+// L2 (buffer_wbl2 sc1) and an agent-scope acquire invalidates it
+// (buffer_inv sc1) -- and the next loads of every wave on that XCD, of
+// adjacency lists and bitmaps too, go to HBM again.  Worklist slots are
+// instead written and read only with agent-coherent accesses (sc1, what
+// relaxed agent-scope atomics compile to), so a hand-off only has to order
+// them: the producer waits until the slot's stores are complete
+// (vmcnt(0)) before it stores the ticket, and the consumer issues the
+// slot's loads after it has seen the ticket; the consumer waits for its
+// loads the same way before it frees the slot.  The workgroup-scope fences
+// only keep the compiler from moving memory accesses across these points.
+__device__ __forceinline__ void handoff_publish()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0): this wave's loads and stores are complete
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+}
+__device__ __forceinline__ void handoff_observe() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+
 // Sleeps 2^min(step, cap) x 512 cycles.
 __device__ __forceinline__ void backoff(uint32_t step, uint32_t cap = 6u)
 {
     uint32_t reps = 1u << (step < cap ? step : cap);
+    for (uint32_t i = 0; i < reps; ++i) __builtin_amdgcn_s_sleep(8);
+}
+
+// Sleep of an idle worker between polls of the worklist.  Every poll reads
+// launch-wide words that all workers share, so with thousands of idle
+// workers the polls themselves slow down the few busy ones (a search of
+// 5,000 tree nodes took 4.2 ms on 7,168 workers and 0.57 ms on 512).  The
+// interval therefore grows with the number of waiting workers -- about one
+// s_sleep(8) unit (~0.2 us) per 16 of them, i.e. a fixed total poll rate --
+// with a jitter from the clock's low bits so that workers that went idle
+// together do not poll together; kMaxIdleReps bounds how long a worker
+// takes to see the end of the launch.
+constexpr uint32_t kMaxIdleReps = 256u;
+__device__ __forceinline__ void backoff_idle(uint32_t step, uint32_t cap, uint32_t waiting)
+{
+    uint32_t reps = 1u << (step < cap ? step : cap);
+    uint32_t share = waiting >> 4;
+    if (share > kMaxIdleReps) share = kMaxIdleReps;
+    if (share > reps) {
+        const uint32_t r = (uint32_t)wall_clock64() & 0xFFu;  // uniform in [share/2, 3 share/2)
+        reps = share / 2u + ((share * r) >> 8);
+    }
     for (uint32_t i = 0; i < reps; ++i) __builtin_amdgcn_s_sleep(8);
 }
 
@@ -1002,13 +1044,100 @@ __device__ __forceinline__ void copy_state(char *__restrict__ dst, const char *_
     }
 }
 
+// Agent-coherent 16-byte accesses to a worklist slot: buffer loads and
+// stores with the sc1 cache policy (the policy relaxed agent-scope atomics
+// use), through a buffer resource that spans the slot.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kCoherent = 16;                 // cache policy bits: sc1
+constexpr int kRawBufferFlags = 0x00020000;   // gfx9 buffer resource word 3: 32-bit data format
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(const void *slot, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(slot), 0, (int)bytes, kRawBufferFlags);
+}
+
+// Node state into a worklist slot (shared) from the worker's own stack slot
+// or node storage (src); the same vectors as copy_state.
+template <class N>
+__device__ __forceinline__ void copy_state_to_ring(__amdgpu_buffer_rsrc_t dst, const char *__restrict__ src,
+                                                   uint32_t npad, uint32_t cnt)
+{
+    const uint32_t q = (cnt + 3u) >> 2;
+    const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
+    const uint32_t o1 = npad >> 2;
+    if constexpr (N::kPacked) {
+        const uint32_t total = 2u * q;
+        for (uint32_t k0 = lane_id(); k0 < total; k0 += kCopyVecs * kWave) {
+            u32x4 x[kCopyVecs];
+#pragma unroll
+            for (uint32_t j = 0; j < kCopyVecs; ++j) {
+                const uint32_t k = k0 + j * kWave;
+                if (k < total) x[j] = s4[k < q ? k : o1 + (k - q)];
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kCopyVecs; ++j) {
+                const uint32_t k = k0 + j * kWave;
+                if (k < total) __builtin_amdgcn_raw_buffer_store_b128(x[j], dst, 16u * (k < q ? k : o1 + (k - q)), 0, kCoherent);
+            }
+        }
+    } else {
+        const uint32_t o2 = npad >> 1;
+        for (uint32_t i = lane_id(); i < q; i += kWave) {
+            const u32x4 x = s4[i], y = s4[o1 + i], z0 = s4[o2 + 2u * i], z1 = s4[o2 + 2u * i + 1u];
+            __builtin_amdgcn_raw_buffer_store_b128(x, dst, 16u * i, 0, kCoherent);
+            __builtin_amdgcn_raw_buffer_store_b128(y, dst, 16u * (o1 + i), 0, kCoherent);
+            __builtin_amdgcn_raw_buffer_store_b128(z0, dst, 16u * (o2 + 2u * i), 0, kCoherent);
+            __builtin_amdgcn_raw_buffer_store_b128(z1, dst, 16u * (o2 + 2u * i + 1u), 0, kCoherent);
+        }
+    }
+}
+
+// Node state from a worklist slot (shared) into the worker's HBM node
+// storage.
+template <class N>
+__device__ __forceinline__ void copy_state_from_ring(char *__restrict__ dst, __amdgpu_buffer_rsrc_t src, uint32_t npad,
+                                                     uint32_t cnt)
+{
+    const uint32_t q = (cnt + 3u) >> 2;
+    u32x4 *d4 = reinterpret_cast<u32x4 *>(dst);
+    const uint32_t o1 = npad >> 2;
+    if constexpr (N::kPacked) {
+        const uint32_t total = 2u * q;
+        for (uint32_t k0 = lane_id(); k0 < total; k0 += kCopyVecs * kWave) {
+            u32x4 x[kCopyVecs];
+#pragma unroll
+            for (uint32_t j = 0; j < kCopyVecs; ++j) {
+                const uint32_t k = k0 + j * kWave;
+                if (k < total) x[j] = __builtin_amdgcn_raw_buffer_load_b128(src, 16u * (k < q ? k : o1 + (k - q)), 0, kCoherent);
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kCopyVecs; ++j) {
+                const uint32_t k = k0 + j * kWave;
+                if (k < total) d4[k < q ? k : o1 + (k - q)] = x[j];
+            }
+        }
+    } else {
+        const uint32_t o2 = npad >> 1;
+        for (uint32_t i = lane_id(); i < q; i += kWave) {
+            const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(src, 16u * i, 0, kCoherent);
+            const u32x4 y = __builtin_amdgcn_raw_buffer_load_b128(src, 16u * (o1 + i), 0, kCoherent);
+            const u32x4 z0 = __builtin_amdgcn_raw_buffer_load_b128(src, 16u * (o2 + 2u * i), 0, kCoherent);
+            const u32x4 z1 = __builtin_amdgcn_raw_buffer_load_b128(src, 16u * (o2 + 2u * i + 1u), 0, kCoherent);
+            d4[i] = x;
+            d4[o1 + i] = y;
+            d4[o2 + 2u * i] = z0;
+            d4[o2 + 2u * i + 1u] = z1;
+        }
+    }
+}
+
 // A stack or worklist slot (HBM) into LDS node storage, packed layout:
 // every lane's 16-byte vector goes straight from global memory into LDS
 // (global_load_lds_dwordx4: lane i of a load writes 16 bytes at the
 // wave-uniform LDS address + 16 i), with no registers in between, so every
 // load of the copy is in flight at once and the copy waits for one round
 // trip.  Lanes past the copied vectors are masked off and write nothing.
-template <class N>
+// kShared: the slot is a worklist slot, loaded agent-coherent (sc1).
+template <class N, bool kShared = false>
 __device__ __forceinline__ void copy_state_to_lds(char *dst, const char *src, uint32_t npad, uint32_t cnt)
 {
     static_assert(N::kPacked && !N::kHbm, "LDS node storage, packed layout");
@@ -1022,7 +1151,7 @@ __device__ __forceinline__ void copy_state_to_lds(char *dst, const char *src, ui
             const uint32_t k = k0 + lane_id();
             if (k < q)
                 __builtin_amdgcn_global_load_lds(reinterpret_cast<const int4 *>(s) + k,
-                                                 reinterpret_cast<int4 *>(d) + k0, 16, 0, 0);
+                                                 reinterpret_cast<int4 *>(d) + k0, 16, 0, kShared ? 16 : 0);
         }
     }
     __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0): every vector has landed in LDS
@@ -1057,7 +1186,8 @@ __device__ bool try_enqueue(const SolveArgs &a, const N &w, const char *src, uin
             slot = (uint32_t)(pos & (unsigned long long)(a.qsize - 1));
             ticket = (uint32_t)(2ull * (pos / a.qsize));
             uint32_t spins = 0;
-            while (ld_acq(&a.tickets[slot]) != ticket) {
+            // the slot is free once its previous consumer has stored 2 lap
+            while (ld_rlx(&a.tickets[slot]) != ticket) {
                 backoff(spins < 3 ? spins : 3);
                 ++spins;
                 if (past_deadline(a)) {
@@ -1075,12 +1205,16 @@ __device__ bool try_enqueue(const SolveArgs &a, const N &w, const char *src, uin
     if (!ok) return false;
     slot = uni(slot);
     ticket = uni(ticket);
-    fence_acq();
-    copy_state<N>(static_cast<char *>(a.q_deg) + (size_t)slot * slot_bytes, src, a.npad, w.n);
-    if (lane_id() == 0) a.q_meta[slot] = make_uint4(g, d, v, 0u);
-    fence_rel();
+    handoff_observe();
+    copy_state_to_ring<N>(slot_rsrc(static_cast<char *>(a.q_deg) + (size_t)slot * slot_bytes, slot_bytes), src, a.npad,
+                          w.n);
     if (lane_id() == 0) {
-        st_rel(&a.tickets[slot], ticket + 1u);
+        const u32x4 m = {g, d, v, 0u};
+        __builtin_amdgcn_raw_buffer_store_b128(m, slot_rsrc(&a.q_meta[slot], 16u), 0u, 0, kCoherent);
+    }
+    handoff_publish();
+    if (lane_id() == 0) {
+        st_rlx(&a.tickets[slot], ticket + 1u);
         fetch_add(&a.glob->counter, kEnqOne);
     }
     return true;
@@ -1088,7 +1222,7 @@ __device__ bool try_enqueue(const SolveArgs &a, const N &w, const char *src, uin
 
 // Takes one item if the occupancy guard says one is there.  On success the
 // degree array is in the worker's node storage and (g, d, v) are returned.
-template <class N>
+template <bool SERVE, class N>
 __device__ bool try_dequeue(const SolveArgs &a, const N &w, uint32_t &g, uint32_t &d, uint32_t &v, uint32_t slot_bytes)
 {
     uint32_t ok = 0, slot = 0, ticket = 0;
@@ -1100,7 +1234,7 @@ __device__ bool try_dequeue(const SolveArgs &a, const N &w, uint32_t &g, uint32_
             slot = (uint32_t)(pos & (unsigned long long)(a.qsize - 1));
             ticket = (uint32_t)(2ull * (pos / a.qsize) + 1ull);
             uint32_t spins = 0;
-            while (ld_acq(&a.tickets[slot]) != ticket) {
+            while (ld_rlx(&a.tickets[slot]) != ticket) {
                 backoff(spins < 3 ? spins : 3);
                 ++spins;
                 if (past_deadline(a)) {
@@ -1116,21 +1250,27 @@ __device__ bool try_dequeue(const SolveArgs &a, const N &w, uint32_t &g, uint32_
     if (!ok) return false;
     slot = uni(slot);
     ticket = uni(ticket);
-    fence_acq();
-    uint4 meta = a.q_meta[slot];
-    g = uni(meta.x);
-    d = uni(meta.y);
-    v = uni(meta.z);
-    if constexpr (N::kPacked && !N::kHbm)
-        copy_state_to_lds<N>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, a.npad,
-                             uni(a.gdesc[g].n));
+    // serving: the request's descriptor and graph were written by the worker
+    // that claimed it (load_request, agent-scope release); they are read
+    // after an agent-scope acquire.  Batch graphs are read-only.
+    if constexpr (SERVE)
+        fence_acq();
     else
-        copy_state<N>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, a.npad,
-                      uni(a.gdesc[g].n));
+        handoff_observe();
+    const u32x4 m = __builtin_amdgcn_raw_buffer_load_b128(slot_rsrc(&a.q_meta[slot], 16u), 0u, 0, kCoherent);
+    g = uni(m.x);
+    d = uni(m.y);
+    v = uni(m.z);
+    if constexpr (N::kPacked && !N::kHbm)
+        copy_state_to_lds<N, true>(w.base, static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, a.npad,
+                                   uni(a.gdesc[g].n));
+    else
+        copy_state_from_ring<N>(w.base, slot_rsrc(static_cast<const char *>(a.q_deg) + (size_t)slot * slot_bytes, slot_bytes),
+                                a.npad, uni(a.gdesc[g].n));
     w.sync();
-    fence_rel();
+    handoff_publish();  // the slot's loads are complete before it is freed
     if (lane_id() == 0) {
-        st_rel(&a.tickets[slot], ticket + 1u);
+        st_rlx(&a.tickets[slot], ticket + 1u);
         // leave "waiting" and take the item off "enqueued" in one step
         fetch_add(&a.glob->counter, (unsigned long long)(0ull - (kEnqOne + kWaitOne)));
     }
@@ -1240,8 +1380,11 @@ __device__ void finish_unit(const SolveArgs &a, uint32_t g, const GDesc &gd)
     GState *gs = &a.gstate[g];
     uint32_t last = 0;
     if (lane_id() == 0) {
-        uint32_t old = __hip_atomic_fetch_add(&gs->outstanding, 0xFFFFFFFFu, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+        // serving: the last finisher reads the cover other workers recorded
+        // (acquire); a batch result is read by the host after the launch
+        uint32_t old = SERVE ? __hip_atomic_fetch_add(&gs->outstanding, 0xFFFFFFFFu, __ATOMIC_ACQ_REL,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                             : fetch_add(&gs->outstanding, 0xFFFFFFFFu);
         last = old == 1u;
         if (last) {
             st_rlx(&gs->done_tick, (uint64_t)wall_clock64());
@@ -1345,8 +1488,23 @@ struct Acquired {
     uint32_t kind, g, d, v, root;
 };
 
+// Work for an idle worker: a queued subtree (kind 1), a root (kind 2: a
+// graph, a static path, or a served request), or nothing (kind 0: the
+// launch is over).  A worker counts as "waiting" in the termination word
+// while it is in here; `counted` says the caller already added it (a worker
+// registers as registered-and-waiting in one atomic at its start).
+//
+// Batch launches count their roots in the termination word's "enqueued"
+// field from the start (init_kernel), like items of the worklist: a claim
+// takes the root index from next_root and then leaves "waiting" and
+// "enqueued" in one atomic, so termination is simply waiting == registered
+// with nothing enqueued, and a failed claim undoes nothing.  At a worker's
+// start only the first `roots` workers to register try to claim roots (the
+// others join in after two polls): with one root and 7,168 workers every
+// worker's claim attempt on the same words cost more than the search of a
+// small graph.
 template <bool SERVE, class N>
-__device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
+__device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes, bool counted, uint32_t rid)
 {
     Acquired r = {0u, kNone, 0u, kNone, 0u};
     // A cancelled or late launch hands out no more work.  Checked on every
@@ -1356,12 +1514,12 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
     uint32_t late = 0;
     if (lane_id() == 0) late = ld_rlx(&a.glob->cancel) || past_deadline(a);
     if (uni(late)) return r;
-    if (lane_id() == 0) fetch_add(&a.glob->counter, kWaitOne);
+    if (!counted && lane_id() == 0) fetch_add(&a.glob->counter, kWaitOne);
     uint32_t spins = 0;
     for (;;) {
         if (a.version == 1) {
             uint32_t g, d, v;
-            if (try_dequeue(a, w, g, d, v, slot_bytes)) {
+            if (try_dequeue<SERVE>(a, w, g, d, v, slot_bytes)) {
                 r.kind = 1;
                 r.g = g;
                 r.d = d;
@@ -1369,17 +1527,28 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
                 return r;
             }
         }
-        const uint32_t avail = roots_available<SERVE>(a);
-        if (uni(ld_rlx(&a.glob->next_root)) < avail) {
-            uint32_t idx = kNone;
-            if (lane_id() == 0) {
-                fetch_add(&a.glob->counter, (unsigned long long)(0ull - kWaitOne));
-                if constexpr (!SERVE) {
-                    // batch: indices past the fixed root count are simply invalid
+        if constexpr (!SERVE) {
+            if ((rid < a.roots || spins >= 2u) && uni(ld_rlx(&a.glob->next_root)) < a.roots) {
+                uint32_t idx = kNone;
+                if (lane_id() == 0) {
                     idx = fetch_add(&a.glob->next_root, 1u);
-                } else {
-                    // serving: never move the counter past the published tail,
-                    // or a request published later would be skipped
+                    if (idx < a.roots) fetch_add(&a.glob->counter, (unsigned long long)(0ull - (kEnqOne + kWaitOne)));
+                }
+                idx = uni(idx);
+                if (idx < a.roots) {
+                    r.kind = 2;
+                    r.root = idx;
+                    return r;
+                }
+            }
+        } else {
+            const uint32_t avail = roots_available<SERVE>(a);
+            if (uni(ld_rlx(&a.glob->next_root)) < avail) {
+                uint32_t idx = kNone;
+                if (lane_id() == 0) {
+                    fetch_add(&a.glob->counter, (unsigned long long)(0ull - kWaitOne));
+                    // never move the counter past the published tail, or a
+                    // request published later would be skipped
                     uint32_t cur = ld_rlx(&a.glob->next_root);
                     while (cur < avail) {
                         if (__hip_atomic_compare_exchange_strong(&a.glob->next_root, &cur, cur + 1u, __ATOMIC_RELAXED,
@@ -1388,14 +1557,10 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
                             break;
                         }
                     }
+                    if (idx >= avail) fetch_add(&a.glob->counter, kWaitOne);
                 }
-                if (idx >= avail) fetch_add(&a.glob->counter, kWaitOne);
-            }
-            idx = uni(idx);
-            if (idx < avail) {
-                r.kind = 2;
-                r.root = idx;
-                if constexpr (SERVE) {
+                idx = uni(idx);
+                if (idx < avail) {
                     // the host wrote the ring entry, descriptor and graph to
                     // pinned memory before publishing the tail
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
@@ -1403,19 +1568,23 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
                     if (lane_id() == 0)
                         slot = __hip_atomic_load(&a.req_ring[idx & a.ring_mask], __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_SYSTEM);
+                    r.kind = 2;
                     r.root = uni(slot);
                     load_request(a, r.root);
+                    return r;
                 }
-                return r;
             }
         }
         unsigned long long c = 0;
         if (lane_id() == 0) c = ld_rlx(&a.glob->counter);
         uint32_t lo = uni((uint32_t)c), hi = uni((uint32_t)(c >> 32));
         uint32_t waiting = lo & 0xFFFFu, reg = lo >> 16;
-        const bool no_more_roots = SERVE ? (uni(ld_rlx(&a.glob->host_shutdown)) != 0u &&
-                                                uni(ld_rlx(&a.glob->next_root)) >= avail)
-                                             : uni(ld_rlx(&a.glob->next_root)) >= a.roots;
+        // batch: unclaimed roots are in `hi`; serving: no root is coming once
+        // the host has shut down and every published request is taken
+        bool no_more_roots = true;
+        if constexpr (SERVE)
+            no_more_roots = uni(ld_rlx(&a.glob->host_shutdown)) != 0u &&
+                            uni(ld_rlx(&a.glob->next_root)) >= roots_available<SERVE>(a);
         if (waiting == reg && hi == 0u && no_more_roots) {
             if (lane_id() == 0) st_rlx(&a.glob->done_flag, 1u);
             return r;
@@ -1423,7 +1592,7 @@ __device__ Acquired acquire(const SolveArgs &a, const N &w, uint32_t slot_bytes)
         if (uni(ld_rlx(&a.glob->done_flag)) || uni(ld_rlx(&a.glob->cancel))) return r;
         if (lane_id() == 0) late = past_deadline(a);
         if (uni(late)) return r;
-        backoff(++spins, a.idle_cap);
+        backoff_idle(++spins, a.idle_cap, waiting);
     }
 }
 
@@ -1502,14 +1671,20 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
     char *stk = static_cast<char *>(a.stk) + (size_t)wid * a.depth * slot_bytes;
     uint2 *stk_meta = a.stk_meta + (size_t)wid * a.depth;
 
+    // register as a worker that waits for work, in one atomic; rid = the
+    // number of workers registered before this one
+    uint32_t rid = 0;
     if (lane_id() == 0) {
-        fetch_add(&a.glob->counter, kRegOne);
+        rid = (uint32_t)(fetch_add(&a.glob->counter, kRegOne + kWaitOne) >> 16) & 0xFFFFu;
         // the time limit counts from the first worker's start, not from the
         // launch: a persistent kernel can wait for CUs held by another one
         uint64_t zero = 0;
-        __hip_atomic_compare_exchange_strong(&a.glob->t0, &zero, (uint64_t)wall_clock64(),
-                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ld_rlx(&a.glob->t0) == 0u)
+            __hip_atomic_compare_exchange_strong(&a.glob->t0, &zero, (uint64_t)wall_clock64(),
+                                                 __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    rid = uni(rid);
+    bool counted = true;  // the first acquire: already counted as waiting
 
     uint32_t g = kNone;
     GDesc gd = {};
@@ -1557,7 +1732,8 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 }
                 if (lane_id() == 0 && ld_rlx(&a.glob->cancel)) st_rlx(&a.glob->done_flag, 1u);
                 const uint64_t t_idle = wall_clock64();
-                Acquired q = acquire<SERVE>(a, w, slot_bytes);
+                Acquired q = acquire<SERVE>(a, w, slot_bytes, counted, rid);
+                counted = false;
                 idle_ticks += wall_clock64() - t_idle;
                 if (q.kind == 0) break;
                 steals += q.kind == 1;
@@ -1788,7 +1964,9 @@ __host__ __device__ inline vc_result result_of(const GState &s, uint32_t instanc
     return r;
 }
 
-__global__ void init_kernel(SolveArgs a, uint32_t units)
+// Resets a launch's scratch.  Batch launches (preload_roots = 1) start
+// with every root counted as enqueued in the termination word (acquire).
+__global__ void init_kernel(SolveArgs a, uint32_t units, uint32_t preload_roots)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -1803,6 +1981,7 @@ __global__ void init_kernel(SolveArgs a, uint32_t units)
     for (uint32_t i = tid; i < a.workers; i += stride) a.wstats[i] = WorkerStats{};
     if (tid == 0) {
         Global z = {};  // t0 = 0: the first solve worker to start sets it
+        if (preload_roots) z.counter = (unsigned long long)a.roots * kEnqOne;
         *a.glob = z;
     }
 }
@@ -2647,7 +2826,7 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
 
     // the device work of one solve, in stream order
     auto enqueue_dispatch = [&](bool timed) -> bool {
-        hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, ctx->stream, a, pl.units);
+        hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, ctx->stream, a, pl.units, 1u);
         if (!hip_ok(hipGetLastError())) return false;
         if (timed && !hip_ok(hipEventRecord(ctx->ev0, ctx->stream))) return false;
         if (!launch_solve(pl, ctx->stream, a)) return false;
@@ -2777,7 +2956,7 @@ vc_status vc_solve_noalloc(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jo
     auto *d_res = reinterpret_cast<vc_result *>(base + tail_at + 256u);
     auto *d_cov = reinterpret_cast<uint8_t *>(base + tail_at + 256u + res_bytes);
 
-    hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, ctx->stream, a, pl.units);
+    hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, ctx->stream, a, pl.units, 1u);
     if (!hip_ok(hipGetLastError()) || !hip_ok(hipEventRecord(ctx->ev0, ctx->stream)) ||
         !launch_solve(pl, ctx->stream, a) || !hip_ok(hipEventRecord(ctx->ev1, ctx->stream)))
         return VC_EDEVICE;
@@ -3268,7 +3447,7 @@ vc_status vc_server_start(vc_ctx *ctx, const vc_server_config *cfg, vc_server **
     // 220 us, and with thousands of them polling at random phases a new
     // request is still seen within the 2 us host-poll interval.
     a.idle_cap = 10u;
-    hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, s->kstream, a, 1u);
+    hipLaunchKernelGGL(vcg::init_kernel, dim3(256), dim3(256), 0, s->kstream, a, 1u, 0u);
     ok = hip_ok(hipGetLastError()) && hip_ok(hipEventRecord(s->ev0, s->kstream)) &&
          launch_solve(s->pl, s->kstream, a) && hip_ok(hipEventRecord(s->ev1, s->kstream));
     if (!ok) {

@@ -315,6 +315,24 @@ def test_autotuner_picks_a_working_shape(ctx):
         assert r.solved and r.minimum == O.mvc(O.from_edges(n, e))[0], name
 
 
+def test_launch_shape_limits(ctx):
+    """Workgroup sizes outside 64..1024 or not a multiple of the wave, and
+    grids of more than 65535 workers (the termination word counts workers in
+    16-bit fields), are refused with VC_EINVAL before anything is launched;
+    1024-thread workgroups (16 workers each) solve."""
+    g = vcgpu.csr_from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0)])
+    gs = ctx.upload(g)
+    job = vcgpu.Job(vcgpu.VC_MVC, 0, 4, 0)
+    for bad in (dict(block_dim=96), dict(block_dim=1088), dict(block_dim=256, num_blocks=16384),
+                dict(block_dim=1024, num_blocks=4096), dict(block_dim=64, num_blocks=65536)):
+        with pytest.raises(vcgpu.VCError) as ei:
+            ctx.solve(gs, job, P(**bad))
+        assert ei.value.status == vcgpu.VC_EINVAL, bad
+    for ok in (dict(block_dim=1024), dict(block_dim=1024, num_blocks=4095), dict(block_dim=64, num_blocks=65535)):
+        (r,), info, _ = ctx.solve(gs, job, P(**ok))
+        assert r.solved and r.minimum == 3, ok
+
+
 def test_upload_rejects_what_the_kernels_cannot_solve(ctx):
     """The kernels rely on sorted, symmetric adjacency (binary search,
     neighbour sums): an edge listed from one end only, or an unsorted list,
