@@ -171,6 +171,11 @@ struct SolveArgs {
     // 1 = every graph of the launch has 2m < 2^16 and node storage is in
     // LDS: a worker keeps its current graph's srcPtr there as uint16
     uint32_t off16;
+    // staged start of launches with few roots: workers wid >= active0 first
+    // sleep about dormant_ticks and join only while the running ones are
+    // nearly all busy (0 = every worker starts at once)
+    uint32_t active0;
+    uint64_t dormant_ticks;
 };
 
 }  // namespace vcg
