@@ -1351,24 +1351,53 @@ __device__ void record_cover(const SolveArgs &a, const N &w, const GDesc &gd, ui
     }
 }
 
+// Nodes a worker processes on graph g before it adds them to g's count.
+// Without a budget: kFlushEvery.  With one, a worker's unflushed nodes
+// times twice the graph's outstanding units stay below what its last view of
+// the count (`seen`) leaves of the budget, so a graph stops within a few
+// percent of its budget however many workers share it.  (With fixed
+// 1,024-node flushes, graphs that many workers shared at the end of a batch
+// ran up to 3.5 times their budget.)  Lane 0's value; the caller makes it
+// uniform.
+constexpr uint32_t kFlushEvery = 1024;
+__device__ __forceinline__ uint32_t flush_interval(const SolveArgs &a, const GDesc &gd, uint32_t g, uint64_t seen)
+{
+    if (!gd.max_nodes) return kFlushEvery;
+    if (seen >= gd.max_nodes) return 1u;
+    // (left / 2) / units, rounded down to a power of two of units
+    const uint32_t units = max(ld_rlx(&a.gstate[g].outstanding), 1u);
+    const uint32_t shift = 1u + (units > 1u ? 32u - (uint32_t)__builtin_clz(units - 1u) : 0u);
+    const uint64_t k = (gd.max_nodes - seen) >> shift;
+    return k >= kFlushEvery ? kFlushEvery : (k > 0u ? (uint32_t)k : 1u);
+}
+
 // Adds the nodes spent on graph g since the last flush to its count (and
 // stops it at its budget), and the HBM bytes of the slot copies made for it
-// (bytes_per_copy each) to the launch total.
+// (bytes_per_copy each) to the launch total.  Returns the next flush
+// interval when there was something to add.
 __device__ __forceinline__ void flush_nodes(const SolveArgs &a, const GDesc &gd, uint32_t g, uint64_t &pending,
-                                            uint32_t &copies, uint32_t bytes_per_copy)
+                                            uint32_t &copies, uint32_t bytes_per_copy, uint32_t &flush_every,
+                                            bool unit_end)
 {
-    if (lane_id() == 0 && copies != 0u)
-        fetch_add(&a.glob->state_bytes, (unsigned long long)copies * bytes_per_copy);
-    copies = 0;
+    // copy bytes go to one launch-wide word: added at the end of a unit or
+    // every 256 copies, not at every (near a budget, frequent) node flush
+    if (unit_end || copies >= 256u) {
+        if (lane_id() == 0 && copies != 0u)
+            fetch_add(&a.glob->state_bytes, (unsigned long long)copies * bytes_per_copy);
+        copies = 0;
+    }
     if (pending == 0) return;
+    uint32_t next = 0;
     if (lane_id() == 0) {
         GState *gs = &a.gstate[g];
-        uint64_t before = fetch_add(&gs->nodes, (uint64_t)pending);
-        if (gd.max_nodes && before + pending >= gd.max_nodes) {
+        const uint64_t after = fetch_add(&gs->nodes, (uint64_t)pending) + pending;
+        if (gd.max_nodes && after >= gd.max_nodes) {
             st_rlx(&gs->budget_hit, 1u);
             st_rlx(&gs->stop, 1u);
         }
+        next = flush_interval(a, gd, g, after);
     }
+    flush_every = uni(next);
     pending = 0;
 }
 
@@ -1663,8 +1692,6 @@ __device__ __forceinline__ GDesc load_desc(const GDesc *p)
 // the persistent solve kernel
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t kFlushEvery = 1024;
-
 // The kernel body takes the launch arguments by reference from a one-line
 // kernel: hipcc then allocates 55 instead of 81 VGPRs for the int16 LDS
 // variant (same source otherwise), so the LDS budget alone sets occupancy.
@@ -1743,6 +1770,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
     // stack pushes and pops, steals (ring slot in) and donations (stack slot
     // to ring slot, two); flush_nodes adds their HBM bytes to the launch total
     uint32_t copies = 0;
+    uint32_t flush_every = kFlushEvery;  // see flush_interval
 
     for (;;) {
         if (!have) {
@@ -1770,7 +1798,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
             } else {
                 if (g != kNone) {
                     total_nodes += pending;
-                    flush_nodes(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n));
+                    flush_nodes(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n), flush_every, true);
                     finish_unit<SERVE>(a, g, gd);
                     g = kNone;
                 }
@@ -1788,6 +1816,11 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                     g = a.version == 1 ? q.root : (q.root >> a.start_depth);
                 }
                 gd = load_desc(a.gdesc + g);
+                {
+                    uint32_t fe = 0;
+                    if (lane_id() == 0) fe = flush_interval(a, gd, g, gd.max_nodes ? ld_rlx(&a.gstate[g].nodes) : 0u);
+                    flush_every = uni(fe);
+                }
                 w.n = gd.n;
                 w.src = a.gbuf + gd.off_src;
                 if (OFF16 && g != off16_g) {
@@ -1830,9 +1863,9 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
             have = false;
             continue;
         }
-        if (++pending >= kFlushEvery) {
+        if (++pending >= flush_every) {
             total_nodes += pending;
-            flush_nodes(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n));
+            flush_nodes(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n), flush_every, false);
             uint32_t stop_all = 0;
             if (lane_id() == 0) stop_all = ld_rlx(&a.glob->cancel) || past_deadline(a);
             if (uni(stop_all)) {
