@@ -533,7 +533,9 @@ def main():
     ap.add_argument("--large-blocks", type=int, default=32,
                     help="served: one-wave workgroups of the large-graph server (each holds an exact-depth stack, "
                          "about 2 GB for BA n=20000)")
-    ap.add_argument("--large-budget", type=int, default=1 << 12, help="served: node budget of a large request")
+    ap.add_argument("--large-budget", type=int, default=1 << 12,
+                    help="served: node budget of a large request (a PVC request at k = U reaches its first cover "
+                         "within 2^15 nodes shared by the server's workers, not within 2^12)")
     ap.add_argument("--large-graphs", type=int, default=4, help="served: distinct BA n=20000 graphs")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--base-seed", type=int, default=1000)
