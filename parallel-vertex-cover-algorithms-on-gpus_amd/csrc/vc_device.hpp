@@ -176,6 +176,9 @@ struct SolveArgs {
     // nearly all busy (0 = every worker starts at once)
     uint32_t active0;
     uint64_t dormant_ticks;
+    // node-budget flushes: workers per graph, the least number of workers
+    // assumed to share a graph (vcg::flush_interval)
+    uint32_t flush_share;
 };
 
 }  // namespace vcg

@@ -699,8 +699,10 @@ template <class N>
 __device__ uint32_t rule_leaf(const N &w, Cands &c)
 {
     uint32_t total = 0;
-    if (!c.full1 && c.n1 == 0u) return 0;  // nothing listed: nothing to mark
     for (;;) {
+        // nothing listed (on entry, or by the last apply): nothing to mark
+        // (a pass per node ended this way, counted on the batched workload)
+        if (!c.full1 && c.n1 == 0u) break;
         uint32_t L = 0;
         if (c.full1 || c.n1 > w.ncap) {
             for (uint32_t base = 0, nv = uni(w.n); base < nv; base += kWave)
@@ -1352,20 +1354,27 @@ __device__ void record_cover(const SolveArgs &a, const N &w, const GDesc &gd, ui
 }
 
 // Nodes a worker processes on graph g before it adds them to g's count.
-// Without a budget: kFlushEvery.  With one, a worker's unflushed nodes
-// times twice the graph's outstanding units stay below what its last view of
-// the count (`seen`) leaves of the budget, so a graph stops within a few
-// percent of its budget however many workers share it.  (With fixed
-// 1,024-node flushes, graphs that many workers shared at the end of a batch
-// ran up to 3.5 times their budget.)  Lane 0's value; the caller makes it
-// uniform.
+// Without a budget: kFlushEvery.  With one, what the worker's last view of
+// the count (`seen`) leaves of the budget, divided by twice the number of
+// workers that may share the graph (rounded to a power of two), so their
+// unflushed nodes stay below half of what is left and a budget-capped graph
+// stops within a few percent of its budget.  (With fixed 1,024-node
+// flushes, graphs that many workers shared at the end of a batch ran up to
+// 3.5 times their budget.)  The sharers: the graph's outstanding units, but
+// at least the launch's workers per graph (a.flush_share) -- a graph can
+// be shared by more workers than it has units at the moment, e.g. when a
+// single graph's search widens or a staged launch's sleeping workers join
+// (counting units alone let one overrun its budget by a quarter).  Dividing
+// by all workers in a batch made every flush 64 nodes and the batched step
+// 4% slower.
 constexpr uint32_t kFlushEvery = 1024;
+template <bool kByWorkers>
 __device__ __forceinline__ uint32_t flush_interval(const SolveArgs &a, const GDesc &gd, uint32_t g, uint64_t seen)
 {
     if (!gd.max_nodes) return kFlushEvery;
     if (seen >= gd.max_nodes) return 1u;
-    // (left / 2) / units, rounded down to a power of two of units
-    const uint32_t units = max(ld_rlx(&a.gstate[g].outstanding), 1u);
+    const uint32_t units = kByWorkers ? max(a.workers, 1u)
+                                      : max(ld_rlx(&a.gstate[g].outstanding), max(a.flush_share, 1u));
     const uint32_t shift = 1u + (units > 1u ? 32u - (uint32_t)__builtin_clz(units - 1u) : 0u);
     const uint64_t k = (gd.max_nodes - seen) >> shift;
     return k >= kFlushEvery ? kFlushEvery : (k > 0u ? (uint32_t)k : 1u);
@@ -1375,6 +1384,7 @@ __device__ __forceinline__ uint32_t flush_interval(const SolveArgs &a, const GDe
 // stops it at its budget), and the HBM bytes of the slot copies made for it
 // (bytes_per_copy each) to the launch total.  Returns the next flush
 // interval when there was something to add.
+template <bool kByWorkers>
 __device__ __forceinline__ void flush_nodes(const SolveArgs &a, const GDesc &gd, uint32_t g, uint64_t &pending,
                                             uint32_t &copies, uint32_t bytes_per_copy, uint32_t &flush_every,
                                             bool unit_end)
@@ -1395,7 +1405,7 @@ __device__ __forceinline__ void flush_nodes(const SolveArgs &a, const GDesc &gd,
             st_rlx(&gs->budget_hit, 1u);
             st_rlx(&gs->stop, 1u);
         }
-        next = flush_interval(a, gd, g, after);
+        next = flush_interval<kByWorkers>(a, gd, g, after);
     }
     flush_every = uni(next);
     pending = 0;
@@ -1798,7 +1808,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
             } else {
                 if (g != kNone) {
                     total_nodes += pending;
-                    flush_nodes(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n), flush_every, true);
+                    flush_nodes<STAGED>(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n), flush_every, true);
                     finish_unit<SERVE>(a, g, gd);
                     g = kNone;
                 }
@@ -1818,7 +1828,8 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 gd = load_desc(a.gdesc + g);
                 {
                     uint32_t fe = 0;
-                    if (lane_id() == 0) fe = flush_interval(a, gd, g, gd.max_nodes ? ld_rlx(&a.gstate[g].nodes) : 0u);
+                    if (lane_id() == 0)
+                        fe = flush_interval<STAGED>(a, gd, g, gd.max_nodes ? ld_rlx(&a.gstate[g].nodes) : 0u);
                     flush_every = uni(fe);
                 }
                 w.n = gd.n;
@@ -1865,7 +1876,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
         }
         if (++pending >= flush_every) {
             total_nodes += pending;
-            flush_nodes(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n), flush_every, false);
+            flush_nodes<STAGED>(a, gd, g, pending, copies, copy_bytes<NodeT>(w.n), flush_every, false);
             uint32_t stop_all = 0;
             if (lane_id() == 0) stop_all = ld_rlx(&a.glob->cancel) || past_deadline(a);
             if (uni(stop_all)) {
@@ -2870,6 +2881,7 @@ SolveArgs solve_args(const vc_ctx *ctx, const vc_graphs *set, const vc_params *p
     // deadline on the device wall clock (every wait loop checks it)
     const uint64_t ms = params->time_limit_ms ? params->time_limit_ms : 3600ull * 1000ull;
     fill_args(a, pl, base, params->version, params->start_depth, ms * (uint64_t)ctx->wall_khz);
+    a.flush_share = std::max<uint32_t>(1u, pl.workers / std::max<uint32_t>(1u, a.num_graphs));
     // staged start (vcg::dormant_join) when the grid is automatic and the
     // roots could keep fewer than half the workers busy
     if (!params->num_blocks && 2ull * pl.roots < pl.workers && kActiveStart < pl.workers) {

@@ -419,3 +419,44 @@ def test_dense_closed_forms(ctx, shape):
         assert r.cover is not None
         cover = np.asarray(r.cover, bool)
         assert int(cover.sum()) == want and bool(np.all(cover[e[:, 0]] | cover[e[:, 1]]))
+
+
+def test_staged_start_long_search_matches_oracle(ctx):
+    """A single graph whose search runs far past the staged start's 0.8 ms
+    (G(150, 0.05) seed 3: 2.8 M tree nodes for the reference's CPU solver),
+    on the automatic grid: the sleeping workers join, and the answer equals
+    the oracle's (whose node count the compiled reference confirmed when the
+    CPU baseline's speed check was made)."""
+    n = 150
+    e = graphgen.erdos_renyi(n, 0.05, 3)
+    og = O.from_edges(n, e)
+    U = O.remove_max(og)
+    want, _, _ = O.mvc(og)
+    g = vcgpu.csr_from_edges(n, e)
+    gs = ctx.upload(g)
+    (r,), info, _ = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, U, 0), P())
+    assert r.solved and r.minimum == want == 100
+    assert info.workers > 512  # the whole grid was launched; 512 start at once
+
+
+def test_node_budget_is_kept(ctx):
+    """A graph out of reach within its node budget stops within a few percent
+    of that budget however many workers share it (the flush interval shrinks
+    with what is left), and reports VC_ETIMEOUT with an incumbent cover that
+    checks."""
+    n = 260
+    e = graphgen.erdos_renyi(n, 0.12, 11)
+    g = vcgpu.csr_from_edges(n, e)
+    U, _ = vcgpu.upper_bound(g, seed=0)
+    gs = ctx.upload(g)
+    budget = 1 << 20
+    # staged start (512 workers first), then 4,096 workers from the start
+    for params in (P(), P(block_dim=256, num_blocks=1024)):
+        (r,), info, _ = ctx.solve(gs, vcgpu.Job(vcgpu.VC_MVC, 0, U, budget), params, raise_on_error=False)
+        assert not r.solved and r.status == vcgpu.VC_ETIMEOUT
+        assert budget <= r.tree_nodes <= budget * 1.1, (r.tree_nodes, info.workers)
+        assert r.minimum <= U
+        if r.cover is not None:
+            cov = np.asarray(r.cover, bool)
+            ed = np.asarray(e, np.int64).reshape(-1, 2)
+            assert np.all(cov[ed[:, 0]] | cov[ed[:, 1]]) and int(cov.sum()) == r.minimum
