@@ -60,7 +60,10 @@ typedef struct {
     uint32_t k;           /* PVC parameter (-k, src/config.h:211-213) */
     uint32_t upper_bound; /* U; must be the size of some vertex cover */
     uint32_t reserved;
-    uint64_t max_nodes;   /* per-graph tree-node budget, 0 = unlimited */
+    uint64_t max_nodes;   /* per-graph tree-node budget, 0 = unlimited; a graph that reaches
+                             it stops within a few percent of it (workers flush their node
+                             counts more often as the budget runs out) and reports
+                             VC_ETIMEOUT with its incumbent */
 } vc_job;
 
 /* Launch-level parameters: the reference's Config (src/config.h:162-176)
@@ -72,7 +75,12 @@ typedef struct {
                                    explicit workgroup's workers do not fit in LDS their node
                                    state goes to HBM (as with -g 1) */
     uint32_t num_blocks;        /* workgroups (-n); 0 = auto (occupancy x CUs).  At most
-                                   65535 workers (num_blocks x block_dim / 64) per launch */
+                                   65535 workers (num_blocks x block_dim / 64) per launch.
+                                   With 0 and fewer roots (graphs x static paths) than half
+                                   the workers, 512 workers start and the rest join once
+                                   they are nearly all busy and the launch has run 0.8 ms
+                                   (a small search ends before they do); an explicit
+                                   num_blocks starts every worker at once */
     int32_t use_global_memory;  /* -g: -1 auto, 0 degree arrays in LDS, 1 in HBM */
     uint32_t list_size;         /* global worklist slots (-q, power of 2); 0 = auto */
     float list_threshold;       /* donate while queued < threshold*size (-t); <=0 = auto */
@@ -127,7 +135,8 @@ typedef struct {
     uint64_t steals;      /* subtrees taken from the worklist */
     float idle_fraction;  /* share of worker time spent waiting for work */
     float active_ms;      /* device wall clock from the first worker's start to the
-                             last worker's exit (0 from a server) */
+                             last searching worker's exit (workers of a staged start
+                             that never joined do not count; 0 from a server) */
     uint64_t state_bytes; /* HBM bytes of node state the search moved: every stack
                              push and pop, worklist steal, and (twice) donation copies
                              the graph's degree/sketch records, 8 bytes per vertex (16
