@@ -374,6 +374,24 @@ def served_leg(ctx, args, pool_graphs=None, pool_ubs=None, pool_alone=None):
     out, wall, bad = W.score(st, results, arrivals, start)
     lat_all = out["small"]["lat"] + out["large"]["lat"]
     exact = out["small"]["exact"] + out["large"]["exact"]
+    capacity = None
+    if args.capacity_rate > 0:
+        # the same kind of stream offered far faster than it can be served:
+        # the answered rate is the servers' capacity, not the offered load
+        pool = args.count
+        cst = W.build_stream(ctx, n_req=args.requests, rate=args.capacity_rate, base_seed=args.base_seed, pool=pool,
+                             large_graphs=args.large_graphs, small_budget=args.served_budget,
+                             large_budget=args.large_budget, time_limit_ms=args.time_limit_ms, seed=78,
+                             pool_graphs=st.graphs[:pool], pool_ubs=st.ubs[:pool], pool_alone=st.alone[:pool])
+        cres, carr, cstart, _, _, _ = W.run_stream(ctx, cst, small_blocks=args.small_blocks,
+                                                   large_blocks=args.large_blocks, time_limit_ms=args.time_limit_ms)
+        cout, cwall, cbad = W.score(cst, cres, carr, cstart)
+        cl = cout["small"]["lat"] + cout["large"]["lat"]
+        capacity = {"offered_rate": args.capacity_rate, "requests": args.requests, "wall_s": cwall,
+                    "requests_per_s": args.requests / cwall,
+                    "graphs_per_s": (cout["small"]["exact"] + cout["large"]["exact"]) / cwall,
+                    "small_requests_per_s": cout["small"]["n"] / cwall,
+                    "p50_ms": pct(cl, 50), "p99_ms": pct(cl, 99), "mismatches": len(cbad)}
     return {
         "workload": f"served stream: Poisson {args.rate:g} req/s, {args.requests} requests; 95% batch-distribution "
                     f"graphs (MVC, or PVC at k = opt +- 1 with opt from the batch solve; budget "
@@ -389,6 +407,7 @@ def served_leg(ctx, args, pool_graphs=None, pool_ubs=None, pool_alone=None):
         "large_workers": info_l.workers if info_l else 0, "large_stack_depth": depth,
         "tree_nodes": info_s.tree_nodes + (info_l.tree_nodes if info_l else 0),
         "idle_fraction_small": info_s.idle_fraction,
+        "capacity": capacity,
     }
 
 
@@ -537,6 +556,8 @@ def main():
                     help="served: node budget of a large request (a PVC request at k = U reaches its first cover "
                          "within 2^15 nodes shared by the server's workers, not within 2^12)")
     ap.add_argument("--large-graphs", type=int, default=4, help="served: distinct BA n=20000 graphs")
+    ap.add_argument("--capacity-rate", type=float, default=50000.0,
+                    help="served: offered rate of a second stream that saturates the servers (0 = skip)")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--base-seed", type=int, default=1000)
     ap.add_argument("--max-nodes", type=int, default=1 << 20, help="per-graph GPU tree-node budget (batch)")
