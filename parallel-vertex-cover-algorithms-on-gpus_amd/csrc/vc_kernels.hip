@@ -272,7 +272,7 @@ template <typename DegT> constexpr uint32_t node_bytes_per_vertex()
     return sizeof(DegT) == 2 ? 8u : 16u;
 }
 
-template <typename DegT, bool HBM, bool OFF16 = false>
+template <typename DegT, bool HBM, bool OFF16 = false, bool ADJ = false>
 struct Node {
     static constexpr bool kPacked = sizeof(DegT) == 2;
     static constexpr bool kHbm = HBM;
@@ -292,12 +292,23 @@ struct Node {
     const uint32_t *src; // graph srcPtr (HBM)
     uint16_t *off16;     // [n+1] the same offsets in LDS, or nullptr (see off())
     const uint32_t *dst; // graph dst (HBM)
+    uint16_t *adj16;     // [2m] the same adjacency in LDS (ADJ), or nullptr (see nb())
     const uint32_t *bits; // graph adjacency bitmap (HBM), or nullptr
     uint32_t n;
     uint32_t lcap;
 
     __device__ __forceinline__ void sync() const { nsync<HBM>(); }
 
+    // dst[x] of the current graph: from the worker's LDS copy in the ADJ
+    // kernel (launches of few graphs whose adjacency fits beside the node
+    // state), else from HBM
+    __device__ __forceinline__ uint32_t nb(uint32_t x) const
+    {
+        if constexpr (ADJ)
+            return adj16[x];
+        else
+            return dst[x];
+    }
     // srcPtr[i] of the current graph: from the worker's LDS copy in the
     // OFF16 kernel (batch launches whose graphs all have 2m < 2^16), else
     // from HBM
@@ -565,7 +576,7 @@ __device__ __forceinline__ void apply_one(const N &w, uint32_t u0, Cands &c)
         uint32_t t = kNone;
         int old = -1;
         if (x < e0) {
-            t = w.dst[x];
+            t = w.nb(x);
             old = w.drop(t, u0);
         }
         add_cands(w, c, old, t);
@@ -610,7 +621,7 @@ __device__ void apply_list(const N &w, uint32_t L, Cands &c)
                 uint32_t t = kNone;
                 int old = -1;
                 if (x < total) {
-                    t = w.dst[b0 + (uint32_t)x];
+                    t = w.nb(b0 + (uint32_t)x);
                     old = w.drop(t, u0);
                 }
                 add_cands(w, c, old, t);
@@ -633,7 +644,7 @@ __device__ void apply_list(const N &w, uint32_t L, Cands &c)
             uint32_t t = kNone;
             int old = -1;
             if (x < total) {
-                t = w.dst[(uint32_t)(ob + (x - ox))];
+                t = w.nb((uint32_t)(ob + (x - ox)));
                 old = w.drop(t, (uint32_t)ou);
             }
             add_cands(w, c, old, t);
@@ -750,11 +761,12 @@ __device__ uint32_t rule_leaf(const N &w, Cands &c)
 
 // Sorted-adjacency membership test (binarySearch, src/helperFunctions.cuh:21-36),
 // half-open [lo, hi).
-__device__ __forceinline__ bool adj_has(const uint32_t *dst, uint32_t lo, uint32_t hi, uint32_t x)
+template <class N>
+__device__ __forceinline__ bool adj_has(const N &w, uint32_t lo, uint32_t hi, uint32_t x)
 {
     while (lo < hi) {
         uint32_t mid = lo + ((hi - lo) >> 1);
-        uint32_t y = dst[mid];
+        uint32_t y = w.nb(mid);
         if (y == x) return true;
         if (y < x)
             lo = mid + 1;
@@ -786,7 +798,7 @@ __device__ __forceinline__ void triangle_at(const N &w, uint32_t v, int dv, type
             const uint32_t bit = n1 * w.n + n2;
             adj = (w.bits[bit >> 5] >> (bit & 31u)) & 1u;
         } else {
-            adj = adj_has(w.dst, w.off(n2), w.off(n2 + 1), n1);
+            adj = adj_has(w, w.off(n2), w.off(n2 + 1), n1);
         }
         if (!skip && adj) {
             c1 = n1;
@@ -938,7 +950,7 @@ __device__ void take_vertex(const N &w, uint32_t v, Cands &c)
         uint32_t t = kNone;
         int dt = -1;
         if (x < e) {
-            t = w.dst[x];
+            t = w.nb(x);
             dt = w.deg(t);
             // the neighbours of one vertex are distinct: plain updates
             if (dt >= 0) w.drop_plain(t, v, dt);
@@ -963,7 +975,7 @@ __device__ uint32_t take_neighbours(const N &w, uint32_t v, Cands &c)
             uint32_t x = x0 + lane_id();
             uint32_t cand = kNone;
             if (x < e) {
-                uint32_t t = w.dst[x];
+                uint32_t t = w.nb(x);
                 if (w.deg(t) >= 0) cand = t;
             }
             L = append(w, L, cand, false);
@@ -990,7 +1002,7 @@ __device__ void init_root(const N &w, const int32_t *deg0, uint32_t npad)
             dv = deg0[v];
             const uint32_t e = w.off(v + 1);
             for (uint32_t x = w.off(v); x < e; ++x) {
-                const uint32_t u = w.dst[x];
+                const uint32_t u = w.nb(x);
                 s += u;
                 q += (S2)u * (S2)u;
             }
@@ -1705,7 +1717,7 @@ __device__ __forceinline__ GDesc load_desc(const GDesc *p)
 // The kernel body takes the launch arguments by reference from a one-line
 // kernel: hipcc then allocates 55 instead of 81 VGPRs for the int16 LDS
 // variant (same source otherwise), so the LDS budget alone sets occupancy.
-template <typename DegT, bool HBM, bool SERVE, bool OFF16, bool STAGED>
+template <typename DegT, bool HBM, bool SERVE, bool OFF16, bool STAGED, bool ADJ = false>
 __device__ __forceinline__ void solve_body(const SolveArgs &a)
 {
     extern __shared__ int4 smem4[];
@@ -1720,7 +1732,8 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
     char *store = HBM ? a.hbm_nodes + (size_t)wid * a.worker_bytes
                       : reinterpret_cast<char *>(smem4) + (size_t)wib * a.worker_bytes;
     const uint32_t slot_bytes = a.npad * node_bytes_per_vertex<DegT>();
-    using NodeT = Node<DegT, HBM, OFF16>;
+    static_assert(!ADJ || OFF16, "the LDS adjacency sits after the LDS offsets");
+    using NodeT = Node<DegT, HBM, OFF16, ADJ>;
     NodeT w;
     w.base = store;
     if constexpr (NodeT::kPacked) {
@@ -1742,6 +1755,8 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
     w.ncap = a.ncap;
     w.src = nullptr;
     w.off16 = OFF16 ? reinterpret_cast<uint16_t *>(w.cand3 + a.ncap) : nullptr;
+    // the adjacency follows the npad + 2 offset entries
+    w.adj16 = ADJ ? w.off16 + a.npad + 2u : nullptr;
     w.dst = nullptr;
     w.bits = nullptr;
     w.n = 0;
@@ -1837,6 +1852,12 @@ __device__ __forceinline__ void solve_body(const SolveArgs &a)
                 if (OFF16 && g != off16_g) {
                     // a graph this worker has not held last: its offsets into LDS
                     for (uint32_t i = lane_id(); i <= w.n; i += kWave) w.off16[i] = (uint16_t)w.src[i];
+                    if constexpr (ADJ) {
+                        // and its adjacency (2m entries, vertex ids below 2^15)
+                        const uint32_t e2 = uni(w.src[w.n]);
+                        const uint32_t *dst = a.gbuf + gd.off_dst;
+                        for (uint32_t i = lane_id(); i < e2; i += kWave) w.adj16[i] = (uint16_t)dst[i];
+                    }
                     w.sync();
                     off16_g = g;
                 }
@@ -2154,6 +2175,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7))) void 
 {
     solve_body<int16_t, false, false, true, true>(a);
 }
+// Staged, with each worker's current graph's adjacency in LDS as well: for
+// launches of few graphs whose adjacency fits beside the node state (small
+// searches are chains of dependent rule passes, each waiting for adjacency
+// reads; from LDS those take ~100 cycles instead of an L2 or MALL trip).
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7))) void solve_kernel_off16_adj(SolveArgs a)
+{
+    solve_body<int16_t, false, false, true, true, true>(a);
+}
 template __global__ void solve_kernel<int16_t, true, false>(SolveArgs);
 template __global__ void solve_kernel<int32_t, true, false>(SolveArgs);
 template __global__ void solve_kernel<int16_t, false, true>(SolveArgs);
@@ -2357,8 +2386,9 @@ bool csr_is_valid(const vc_csr &g)
 // need at least 512 KB at n = 32768, so they are never LDS-resident.
 enum Variant { kLds16 = 0, kHbm16 = 1, kHbm32 = 2 };
 
-const void *kernel_of(int variant, bool serve, bool off16 = false)
+const void *kernel_of(int variant, bool serve, bool off16 = false, bool adj = false)
 {
+    if (adj) return reinterpret_cast<const void *>(vcg::solve_kernel_off16_adj);
     if (serve) {
         switch (variant) {
         case kLds16: return reinterpret_cast<const void *>(vcg::solve_kernel<int16_t, false, true>);
@@ -2381,6 +2411,7 @@ struct Plan {
         num_blocks, workers;
     uint32_t depth, qsize, qthreshold, roots, units;
     uint32_t off16;  // workers keep their graph's srcPtr in LDS as uint16
+    uint32_t adj;    // ... and its dst (solve_kernel_off16_adj, a staged launch)
     size_t smem;
     size_t off_glob, off_stk, off_meta, off_qdeg, off_qmeta, off_tickets, off_wnodes, off_hbm, total;
 };
@@ -2397,7 +2428,7 @@ constexpr uint64_t kAdjBitsMaxWords = 1ull << 31;
 // Per-worker node storage for a degree type: node state [npad] | list [lcap] | mark [mwords] |
 // xs [64] | cand [ncap] | cand3 [ncap] | (off16) srcPtr [npad + 2] uint16.  A stack / worklist
 // slot is the node state.
-void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = false)
+void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = false, uint32_t adj_entries = 0)
 {
     pl.deg_bytes = deg_bytes;
     pl.npad = round_up(nmax, 64u / deg_bytes);
@@ -2412,7 +2443,7 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = fals
     const uint32_t cand_bytes = 2u * pl.ncap * id_bytes;
     pl.worker_bytes =
         round_up(pl.slot_bytes + id_bytes * pl.lcap + 4u * pl.mwords + 4u * vcg::kWave + cand_bytes +
-                     (off16 ? 2u * (pl.npad + 2u) : 0u),
+                     (off16 ? 2u * (pl.npad + 2u) : 0u) + 2u * round_up(adj_entries, 4u),
                  64u);
 }
 
@@ -2423,7 +2454,7 @@ constexpr uint32_t kMaxBlockDim = 1024;   // the solve kernels' launch bound
 constexpr uint32_t kMaxWorkers = 0xFFFFu; // 16-bit worker counts in the termination word
 
 vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_params *p, Plan &pl, bool serve,
-                      bool off16_ok = false, size_t mem_budget = 0)
+                      bool off16_ok = false, size_t mem_budget = 0, uint32_t adj_entries = 0)
 {
     const uint32_t version = p->version;
     if (version != VC_HYBRID && version != VC_STACK_ONLY) return VC_EINVAL;
@@ -2456,12 +2487,28 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
     // LDS copy of the current graph's offsets (batch launches whose graphs
     // all have 2m < 2^16; a server reuses graph slots for new requests)
     pl.off16 = 0u;
+    pl.adj = 0u;
     if (!serve && off16_ok && pl.variant == kLds16) {
         Plan t = pl;
         size_storage(t, nmax, 2u, true);
         if ((size_t)wpb * t.worker_bytes <= lds_limit) {
             pl = t;
             pl.off16 = 1u;
+        }
+        // and the adjacency (adj_entries = the largest 2m of the launch),
+        // when asked for; an automatic workgroup shrinks to fit
+        if (pl.off16 && adj_entries) {
+            Plan ta = pl;
+            size_storage(ta, nmax, 2u, true, adj_entries);
+            uint32_t wa = wpb;
+            if (!p->block_dim)
+                while (wa > 1 && (size_t)wa * ta.worker_bytes > lds_limit) wa /= 2;
+            if ((size_t)wa * ta.worker_bytes <= lds_limit) {
+                pl = ta;
+                pl.off16 = 1u;
+                pl.adj = 1u;
+                wpb = wa;
+            }
         }
     }
     pl.wpb = wpb;
@@ -2470,8 +2517,8 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
     if (pl.smem > lds_limit) return VC_ENOTSUP;
 
     int bpc = 0;
-    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel_of(pl.variant, serve, pl.off16 != 0u), (int)pl.block_dim,
-                                                             pl.smem)))
+    if (!hip_ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kernel_of(pl.variant, serve, pl.off16 != 0u, pl.adj != 0u),
+                                                             (int)pl.block_dim, pl.smem)))
         return VC_EDEVICE;
     if (bpc < 1) return VC_ENOTSUP;
     pl.blocks_per_cu = (uint32_t)bpc;
@@ -2551,9 +2598,26 @@ vc_status make_plan(vc_ctx *ctx, const vc_graphs *set, const vc_job *jobs, const
         need = std::max(need, lim);
     }
     bool off16_ok = true;
-    for (int i = 0; i < set->count; ++i) off16_ok &= 2ull * set->h_desc[i].m < 65536ull;
+    uint32_t e2max = 0;
+    for (int i = 0; i < set->count; ++i) {
+        off16_ok &= 2ull * set->h_desc[i].m < 65536ull;
+        e2max = std::max<uint32_t>(e2max, (uint32_t)std::min<uint64_t>(2ull * set->h_desc[i].m, 0xFFFFFFFFull));
+    }
     vc_status st = plan_launch(ctx, set->nmax, need + 1u, p, pl, false, off16_ok, mem_budget);
     if (st != VC_OK) return st;
+    // A launch of few roots on the automatic grid (the staged start) also
+    // tries the adjacency in LDS (solve_kernel_off16_adj).  Taken when it
+    // keeps the worker count, or for graphs of 1,024 vertices or more (no
+    // adjacency bitmap: the triangle rule's pair test is a binary search of
+    // the adjacency) -- a single BA n = 2,000 PVC solve, a chain of
+    // dependent rule passes, ran 1.3 ms with global adjacency.
+    const uint64_t roots_est = (uint64_t)set->count * pl.units;
+    if (!p->num_blocks && pl.off16 && e2max > 0 && 2ull * roots_est < pl.workers) {
+        Plan pa;
+        if (plan_launch(ctx, set->nmax, need + 1u, p, pa, false, off16_ok, mem_budget, e2max) == VC_OK && pa.adj &&
+            (pa.workers >= pl.workers || set->nmax >= kAdjBitsMaxN))
+            pl = pa;
+    }
     // roots (graphs x static paths) are counted in the termination word's
     // 32-bit enqueued field
     if ((uint64_t)set->count * pl.units >= (1ull << 31)) return VC_EINVAL;
@@ -2592,6 +2656,10 @@ void fill_args(SolveArgs &a, const Plan &pl, char *base, uint32_t version, uint3
 template <bool SERVE>
 void launch_variant(const Plan &pl, hipStream_t stream, const SolveArgs &a)
 {
+    if (!SERVE && pl.adj) {
+        hipLaunchKernelGGL(vcg::solve_kernel_off16_adj, dim3(pl.num_blocks), dim3(pl.block_dim), pl.smem, stream, a);
+        return;
+    }
     if (!SERVE && a.active0 != 0u) {
         // staged start: the same kernels with the dormant_join prologue
         switch (pl.variant) {
@@ -2883,8 +2951,9 @@ SolveArgs solve_args(const vc_ctx *ctx, const vc_graphs *set, const vc_params *p
     fill_args(a, pl, base, params->version, params->start_depth, ms * (uint64_t)ctx->wall_khz);
     a.flush_share = std::max<uint32_t>(1u, pl.workers / std::max<uint32_t>(1u, a.num_graphs));
     // staged start (vcg::dormant_join) when the grid is automatic and the
-    // roots could keep fewer than half the workers busy
-    if (!params->num_blocks && 2ull * pl.roots < pl.workers && kActiveStart < pl.workers) {
+    // roots could keep fewer than half the workers busy; the adjacency
+    // kernel is always staged (active0 >= workers: every worker starts)
+    if (pl.adj || (!params->num_blocks && 2ull * pl.roots < pl.workers && kActiveStart < pl.workers)) {
         a.active0 = std::max<uint32_t>(kActiveStart, 2u * pl.roots);
         a.dormant_ticks = (uint64_t)ctx->wall_khz / 10u;  // 100 us
     }
