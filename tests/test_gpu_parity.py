@@ -460,3 +460,27 @@ def test_node_budget_is_kept(ctx):
             cov = np.asarray(r.cover, bool)
             ed = np.asarray(e, np.int64).reshape(-1, 2)
             assert np.all(cov[ed[:, 0]] | cov[ed[:, 1]]) and int(cov.sum()) == r.minimum
+
+
+def test_large_sparse_single_graphs_match_oracle(ctx):
+    """Single graphs of 1,024 vertices or more on the automatic grid (the
+    staged launch that keeps the graph's adjacency in LDS, whose triangle
+    rule binary-searches that copy): MVC, and PVC at k = MVC - 1 and MVC,
+    equal the oracle's, with covers that check."""
+    # the oracle settles each in under 100 tree nodes
+    cases = [("ba2000", 2000, graphgen.barabasi_albert(2000, 3, 1)),
+             ("ba3000", 3000, graphgen.barabasi_albert(3000, 3, 2)),
+             ("ba1200", 1200, graphgen.barabasi_albert(1200, 3, 5))]
+    for name, n, e in cases:
+        og = O.from_edges(n, e)
+        want, st, _ = O.mvc(og, max_nodes=100000)
+        assert st["completed"], name
+        g = vcgpu.csr_from_edges(n, e)
+        mvc, cover, r, info = vcgpu.solve_mvc(ctx, g, P())
+        assert r.solved and mvc == want, (name, mvc, want)
+        assert O.cover_is_valid(og, cover) and int(np.sum(cover)) == want, name
+        for k in (want - 1, want):
+            found, pcover, pr, _ = vcgpu.solve_pvc(ctx, g, k, P())
+            assert pr.solved and found == (k >= want), (name, k)
+            if found:
+                assert O.cover_is_valid(og, pcover) and int(np.sum(pcover)) <= k, name
