@@ -2308,6 +2308,7 @@ void *noalloc_scratch(int device)
 struct DispatchKey {
     int variant;
     uint32_t num_blocks, block_dim, version, start_depth, qsize, qthreshold, depth, want_covers;
+    uint32_t off16, adj, active0, roots, flush_share;  // kernel choice and staged start
     size_t smem;
     uint64_t limit_ticks;
     const void *scratch;
@@ -3067,6 +3068,11 @@ vc_status vc_solve(vc_ctx *ctx, const vc_graphs *set_c, const vc_job *jobs, cons
         key.qthreshold = pl.qthreshold;
         key.depth = pl.depth;
         key.want_covers = want_covers;
+        key.off16 = pl.off16;
+        key.adj = pl.adj;
+        key.active0 = a.active0;
+        key.roots = a.roots;
+        key.flush_share = a.flush_share;
         key.smem = pl.smem;
         key.limit_ticks = a.limit_ticks;
         key.scratch = ctx->scratch;
