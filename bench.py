@@ -196,8 +196,9 @@ def traffic_from(fetch_db, write_db, cal_fetch_db, cal_write_db, cal_bytes: int,
 
 
 def b_node(n: int, m: int) -> int:
-    """SURVEY's per-node byte figure, kept as a labelled legacy field: node
-    state in (4n+4), child state out (4n+4), one CSR sweep (4(n+1) + 8m).  It
+    """The reference-shaped per-node byte figure, kept as a labelled legacy
+    field: node state in (4n+4), child state out (4n+4), one CSR sweep
+    (4(n+1) + 8m), as the reference's global-memory kernels move it.  It
     charges every node a full HBM round trip, which this LDS-resident design
     does not make (its achieved rate exceeds the HBM peak)."""
     return 12 * n + 8 * m + 12
@@ -530,8 +531,9 @@ def batch_leg(ctx, args, params, out):
                      "b_node_legacy": {"achieved_gbs": legacy_bytes / kernel_s / 1e9,
                                        "frac": legacy_bytes / kernel_s / 1e9 / HBM_PEAK_GBS,
                                        "algorithmic_bytes": legacy_bytes,
-                                       "note": "SURVEY B_node = 12n+8m+12 per node: charges each node a full "
-                                               "HBM round trip this LDS-resident design does not make"}},
+                                       "note": "B_node = 12n+8m+12 per node (the reference's global-memory "
+                                               "kernels' traffic): charges each node a full HBM round trip "
+                                               "this LDS-resident design does not make"}},
     })
     return graphs, ubs, res
 
