@@ -2957,9 +2957,12 @@ vc_status prepare_solve(vc_ctx *ctx, vc_graphs *set, const vc_job *jobs, const v
 }
 
 // Launch arguments of a batch solve of `set` with scratch at base.
-// Workers that start at once in a staged launch: two per CU.  Measured on a
-// 5,457-node search (3-regular, n = 120): 256 workers 0.48 ms, 512 0.6 ms,
-// 1,024 0.7-1.0 ms, 7,168 2.4-3.0 ms.
+// Workers that start at once in a staged launch: two per CU.  Kernel time
+// with 128 / 256 / 512 starting workers (the rest joining as described at
+// dormant_join): 3-regular n = 120 MVC (5,453 nodes) 0.73-0.74 / 0.66-0.67 /
+// 0.72-0.77 ms; BA n = 2,000 PVC answer 0.57 / 0.57 / 0.46 ms; ER
+// G(200, 0.05) MVC ~210 ms for all three.  With every worker started (7,168)
+// the 3-regular search took 2.4-3.0 ms.
 constexpr uint32_t kActiveStart = 512;
 
 SolveArgs solve_args(const vc_ctx *ctx, const vc_graphs *set, const vc_params *params, const Plan &pl, char *base)
