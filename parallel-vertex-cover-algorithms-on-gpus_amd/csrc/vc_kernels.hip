@@ -2961,8 +2961,11 @@ vc_status prepare_solve(vc_ctx *ctx, vc_graphs *set, const vc_job *jobs, const v
 // with 128 / 256 / 512 starting workers (the rest joining as described at
 // dormant_join): 3-regular n = 120 MVC (5,453 nodes) 0.73-0.74 / 0.66-0.67 /
 // 0.72-0.77 ms; BA n = 2,000 PVC answer 0.57 / 0.57 / 0.46 ms; ER
-// G(200, 0.05) MVC ~210 ms for all three.  With every worker started (7,168)
-// the 3-regular search took 2.4-3.0 ms.
+// G(200, 0.05) MVC ~210 ms for all three.  256 helps only searches under a
+// millisecond (four 3-regular n = 120 graphs: 12-15%) and hurts longer ones
+// that still end before the others join (3-regular n = 160, 36k nodes:
+// 2.25 -> 3.3 ms; ER G(120, 0.08), 199k nodes: 14.9 -> 16.3 ms).  With every
+// worker started (7,168) the n = 120 search took 2.4-3.0 ms.
 constexpr uint32_t kActiveStart = 512;
 
 SolveArgs solve_args(const vc_ctx *ctx, const vc_graphs *set, const vc_params *params, const Plan &pl, char *base)
