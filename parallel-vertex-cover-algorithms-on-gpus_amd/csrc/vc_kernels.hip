@@ -2464,6 +2464,12 @@ struct Plan {
 // entries per rule-candidate list: small enough that 28 workers' LDS fits a
 // CU at n = 400 (a longer run of candidates makes the next pass scan all)
 constexpr uint32_t kCandCap = 64;
+// The same for plans with the adjacency in LDS (launches of few, often large
+// graphs, whose occupancy is already set by that adjacency): on BA n = 2,000
+// a hub's removal and the triangle rule's deletions list hundreds of leaf
+// candidates at once, and with 64 entries every such leaf pass scanned all
+// 2,000 vertices (80% of the search's busy time).
+constexpr uint32_t kCandCapAdj = 1024;
 constexpr uint32_t kListCap = 1024;  // marks per rule pass (the rest go to the next pass)
 // graphs up to this many vertices get an adjacency bitmap (128 KB at most),
 // while the graph set stays below 2^31 words
@@ -2482,7 +2488,7 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = fals
     pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), kListCap);
     // candidate lists: a longer run of candidates between two passes of a
     // rule makes that pass scan every vertex instead
-    pl.ncap = std::min<uint32_t>(round_up(nmax, 4u), kCandCap);
+    pl.ncap = std::min<uint32_t>(round_up(nmax, 4u), adj_entries ? kCandCapAdj : kCandCap);
     const uint32_t id_bytes = deg_bytes == 2u ? 2u : 4u;  // mark and candidate list entries
     pl.mwords = round_up((nmax + 31u) / 32u, 4u);
     const uint32_t cand_bytes = 2u * pl.ncap * id_bytes;
