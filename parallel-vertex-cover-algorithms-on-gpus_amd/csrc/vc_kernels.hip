@@ -2464,12 +2464,12 @@ struct Plan {
 // entries per rule-candidate list: small enough that 28 workers' LDS fits a
 // CU at n = 400 (a longer run of candidates makes the next pass scan all)
 constexpr uint32_t kCandCap = 64;
-// The same for plans with the adjacency in LDS (launches of few, often large
-// graphs, whose occupancy is already set by that adjacency): on BA n = 2,000
-// a hub's removal and the triangle rule's deletions list hundreds of leaf
-// candidates at once, and with 64 entries every such leaf pass scanned all
-// 2,000 vertices (80% of the search's busy time).
-constexpr uint32_t kCandCapAdj = 1024;
+// The same for plans whose occupancy the lists do not set: node state in
+// HBM, or the adjacency in LDS (launches of few, often large graphs).  On
+// BA n = 2,000 a hub's removal and the triangle rule's deletions list
+// hundreds of leaf candidates at once, and with 64 entries every such leaf
+// pass scanned all 2,000 vertices (80% of the search's busy time).
+constexpr uint32_t kCandCapLarge = 1024;
 constexpr uint32_t kListCap = 1024;  // marks per rule pass (the rest go to the next pass)
 // graphs up to this many vertices get an adjacency bitmap (128 KB at most),
 // while the graph set stays below 2^31 words
@@ -2479,7 +2479,8 @@ constexpr uint64_t kAdjBitsMaxWords = 1ull << 31;
 // Per-worker node storage for a degree type: node state [npad] | list [lcap] | mark [mwords] |
 // xs [64] | cand [ncap] | cand3 [ncap] | (off16) srcPtr [npad + 2] uint16.  A stack / worklist
 // slot is the node state.
-void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = false, uint32_t adj_entries = 0)
+void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = false, uint32_t adj_entries = 0,
+                  uint32_t cand_cap = kCandCap)
 {
     pl.deg_bytes = deg_bytes;
     pl.npad = round_up(nmax, 64u / deg_bytes);
@@ -2488,7 +2489,7 @@ void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = fals
     pl.lcap = std::min<uint32_t>(round_up(nmax, 4u), kListCap);
     // candidate lists: a longer run of candidates between two passes of a
     // rule makes that pass scan every vertex instead
-    pl.ncap = std::min<uint32_t>(round_up(nmax, 4u), adj_entries ? kCandCapAdj : kCandCap);
+    pl.ncap = std::min<uint32_t>(round_up(nmax, 4u), adj_entries ? kCandCapLarge : cand_cap);
     const uint32_t id_bytes = deg_bytes == 2u ? 2u : 4u;  // mark and candidate list entries
     pl.mwords = round_up((nmax + 31u) / 32u, 4u);
     const uint32_t cand_bytes = 2u * pl.ncap * id_bytes;
@@ -2535,6 +2536,7 @@ vc_status plan_launch(vc_ctx *ctx, uint32_t nmax_in, uint32_t depth, const vc_pa
     }
     if (hbm && p->use_global_memory == 0) return VC_ENOTSUP;
     pl.variant = wide ? kHbm32 : hbm ? kHbm16 : kLds16;
+    if (hbm) size_storage(pl, nmax, wide ? 4u : 2u, false, 0u, kCandCapLarge);
     // LDS copy of the current graph's offsets (batch launches whose graphs
     // all have 2m < 2^16; a server reuses graph slots for new requests)
     pl.off16 = 0u;
