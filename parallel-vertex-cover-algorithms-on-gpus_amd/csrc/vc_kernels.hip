@@ -1709,8 +1709,8 @@ __device__ bool dormant_join(const SolveArgs &a, uint32_t wid)
     }
 }
 
-// count 32-bit words from HBM into 16-bit LDS entries with kInFlight loads
-// per lane issued before the first store: a loop of one load and one store
+// Copies count 32-bit words from HBM into 16-bit LDS entries with kInFlight
+// loads per lane issued before the first store: a loop of one load and one store
 // per lane waits a full memory round trip for every 64 entries (BA n = 2000,
 // m = 3, 12k adjacency entries: a PVC answer 0.485 -> 0.465 ms and its MVC
 // solve 0.77 -> 0.70 ms with 8 in flight).
@@ -2477,8 +2477,8 @@ constexpr uint32_t kAdjBitsMaxN = 1024;
 constexpr uint64_t kAdjBitsMaxWords = 1ull << 31;
 
 // Per-worker node storage for a degree type: node state [npad] | list [lcap] | mark [mwords] |
-// xs [64] | cand [ncap] | cand3 [ncap] | (off16) srcPtr [npad + 2] uint16.  A stack / worklist
-// slot is the node state.
+// xs [64] | cand [ncap] | cand3 [ncap] | (off16) srcPtr [npad + 2] uint16 | (adjacency in LDS)
+// dst [adj_entries] uint16.  A stack / worklist slot is the node state.
 void size_storage(Plan &pl, uint32_t nmax, uint32_t deg_bytes, bool off16 = false, uint32_t adj_entries = 0,
                   uint32_t cand_cap = kCandCap)
 {
@@ -2987,7 +2987,8 @@ vc_status prepare_solve(vc_ctx *ctx, vc_graphs *set, const vc_job *jobs, const v
 // Launch arguments of a batch solve of `set` with scratch at base.
 // Workers that start at once in a staged launch: two per CU.  Kernel time
 // with 128 / 256 / 512 starting workers (the rest joining as described at
-// dormant_join): 3-regular n = 120 MVC (5,453 nodes) 0.73-0.74 / 0.66-0.67 /
+// dormant_join; measured before its end-only look, which took the 512 case
+// of the first graph to 0.64-0.66 ms): 3-regular n = 120 MVC (5,453 nodes) 0.73-0.74 / 0.66-0.67 /
 // 0.72-0.77 ms; BA n = 2,000 PVC answer 0.57 / 0.57 / 0.46 ms; ER
 // G(200, 0.05) MVC ~210 ms for all three.  256 helps only searches under a
 // millisecond (four 3-regular n = 120 graphs: 12-15%) and hurts longer ones
