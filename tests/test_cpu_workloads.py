@@ -4,6 +4,7 @@ answer, cover that does not check, or incumbent above U is a mismatch.
 No GPU: the results are constructed here."""
 import numpy as np
 
+import graphgen
 import vcgpu
 import workloads as W
 
@@ -72,3 +73,49 @@ def test_cover_checks():
     assert W.cover_checks(e, [1, 1, 0], k=2) and not W.cover_checks(e, [1, 1, 1], k=2)
     assert not W.cover_checks(e, None)
     assert W.cover_checks(np.zeros((0, 2), np.uint32), [0, 0])
+
+
+def test_stream_composition():
+    # a pool of 24 batch graphs with made-up batch-solve results (every third
+    # one unsolved), one large graph; no device is needed when the pool's
+    # results are given
+    pool = 24
+    small = [graphgen.batch_graph(i, 1000)[:2] for i in range(pool)]
+    graphs = [vcgpu.csr_from_edges(n, e) for n, e in small]
+    ubs = [vcgpu.upper_bound(g, seed=1000) for g in graphs]
+    alone = [_res(u - 1, solved=(i % 3 != 0)) for i, (u, _) in enumerate(ubs)]
+    n_req, rate = 2000, 1000.0
+    st = W.build_stream(None, n_req, rate, pool=pool, large_graphs=1, small_budget=1 << 16, large_budget=1 << 12,
+                        pool_graphs=graphs, pool_ubs=ubs, pool_alone=alone)
+    again = W.build_stream(None, n_req, rate, pool=pool, large_graphs=1, small_budget=1 << 16,
+                           large_budget=1 << 12, pool_graphs=graphs, pool_ubs=ubs, pool_alone=alone)
+    assert [(r.gi, r.kind, r.job.k) for r in st.requests] == [(r.gi, r.kind, r.job.k) for r in again.requests]
+    assert len(st.requests) == n_req and len(st.gaps) == n_req
+    assert abs(st.gaps.mean() * rate - 1.0) < 0.1          # Poisson arrivals at the stated rate
+    large = [r for r in st.requests if r.cls == "large"]
+    assert 60 <= len(large) <= 140                         # 5% of 2,000 (binomial, > 4 sigma)
+    U_large = st.ubs[pool][0]
+    assert st.graphs[pool].n == W.LARGE_N and len(st.edges[pool]) == W.LARGE_N * W.LARGE_M - W.LARGE_M ** 2
+    for r in large:
+        assert r.gi == pool and r.job.upper_bound == U_large and r.job.max_nodes == 1 << 12
+        if r.kind == "large_mvc":
+            assert r.job.instance == vcgpu.VC_MVC and r.expect is None
+        else:
+            assert r.kind == "large_pvc_at_U" and r.job.instance == vcgpu.VC_PVC
+            assert r.job.k == U_large and r.expect == 1
+    kinds = {}
+    for r in st.requests:
+        if r.cls != "small":
+            continue
+        kinds[r.kind] = kinds.get(r.kind, 0) + 1
+        assert 0 <= r.gi < pool and r.job.max_nodes == 1 << 16 and r.job.upper_bound == ubs[r.gi][0]
+        opt = alone[r.gi].minimum
+        if not alone[r.gi].solved:
+            assert r.kind == "mvc_budget" and r.job.instance == vcgpu.VC_MVC and r.expect is None
+        elif r.kind == "mvc":
+            assert r.job.instance == vcgpu.VC_MVC and r.expect == opt
+        else:
+            # PVC one below (answer no) or one above (answer yes) the known MVC
+            assert r.kind == "pvc" and r.job.instance == vcgpu.VC_PVC
+            assert r.job.k in (opt - 1, opt + 1) and r.expect == int(r.job.k >= opt)
+    assert kinds.get("mvc", 0) > 0 and kinds.get("pvc", 0) > 0 and kinds.get("mvc_budget", 0) > 0
