@@ -59,16 +59,74 @@ vc_status build_csr(uint32_t n, uint32_t m, const uint32_t *eu, const uint32_t *
 // overwriting it with that list's last live entry.  The approximations read
 // the lists after such edits (full original ranges, unsorted prefixes), so
 // the same edit is kept here to obtain the same U.
+//
+// The reference's rule passes scan every vertex (leafReductionRule repeats
+// full scans until one finds no degree-1 vertex; triangleReductionRule
+// rescans from vertex 0 after every triangle it applies), which makes both
+// approximations quadratic in n.  Here each rule keeps a set of candidate
+// vertices and visits only those, in the same index order, with the same
+// per-vertex test: a vertex outside the set is one whose test cannot fire.
+// A test reads the vertex's degree, the degrees of the vertices in its
+// adjacency range, and (triangle) the range of one of them; all of these
+// change only in remove(), which lists every vertex that can be affected:
+// each vertex whose degree changed, and the degree-2 vertices in the
+// adjacency range of each (a live vertex stays in the range of every
+// neighbour, since an edit only overwrites the deleted vertex).  So the
+// rules delete the same vertices in the same order as the full scans.
 struct Approx {
     uint32_t n;
     std::vector<uint32_t> src, dst;
     std::vector<int32_t> deg;
+    std::vector<uint64_t> leaf_set, tri_set;  // candidate bitsets
+    uint32_t leaf_n = 0, tri_n = 0;           // their members
 
     explicit Approx(const vc_csr &g) : n(g.vertexNum)
     {
         src.assign(g.srcPtr, g.srcPtr + n + 1);
         dst.assign(g.dst, g.dst + 2ull * g.edgeNum);
         deg.assign(g.degree, g.degree + n);
+        leaf_set.assign((n + 63u) / 64u, 0ull);
+        tri_set.assign((n + 63u) / 64u, 0ull);
+        for (uint32_t v = 0; v < n; ++v) {
+            if (deg[v] == 1) add(leaf_set, leaf_n, v);
+            if (deg[v] == 2) add(tri_set, tri_n, v);
+        }
+    }
+
+    static void add(std::vector<uint64_t> &s, uint32_t &count, uint32_t v)
+    {
+        const uint64_t bit = 1ull << (v & 63u);
+        if (!(s[v >> 6] & bit)) {
+            s[v >> 6] |= bit;
+            ++count;
+        }
+    }
+    static void take(std::vector<uint64_t> &s, uint32_t &count, uint32_t v)
+    {
+        s[v >> 6] &= ~(1ull << (v & 63u));
+        --count;
+    }
+    // lowest member >= from, or n when there is none
+    uint32_t next(const std::vector<uint64_t> &s, uint32_t count, uint32_t from) const
+    {
+        if (count == 0 || from >= n) return n;
+        uint32_t w = from >> 6;
+        uint64_t bits = s[w] & (~0ull << (from & 63u));
+        while (!bits) {
+            if (++w >= s.size()) return n;
+            bits = s[w];
+        }
+        return std::min<uint32_t>(n, (w << 6) + (uint32_t)__builtin_ctzll(bits));
+    }
+
+    // x's degree changed: x and the degree-2 vertices of its range may now
+    // pass a rule's test
+    void touched(uint32_t x)
+    {
+        if (deg[x] == 1) add(leaf_set, leaf_n, x);
+        if (deg[x] == 2) add(tri_set, tri_n, x);
+        for (uint32_t j = src[x]; j < src[x + 1]; ++j)
+            if (deg[dst[j]] == 2) add(tri_set, tri_n, dst[j]);
     }
 
     void remove(uint32_t v)
@@ -85,6 +143,8 @@ struct Approx {
             }
         }
         deg[v] = -1;
+        touched(v);
+        for (int32_t i = vb; i < ve; ++i) touched(dst[i]);
     }
 
     // Membership search over a range that may be unsorted after edits
@@ -106,15 +166,15 @@ struct Approx {
         return false;
     }
 
-    // leafReductionRule(CSRGraph&, unsigned&) (src/auxFunctions.cpp:44-69)
+    // leafReductionRule(CSRGraph&, unsigned&) (src/auxFunctions.cpp:44-69):
+    // passes in index order; a vertex listed behind a pass's position is
+    // visited by the next pass, as the next full scan would find it
     void leaf(uint32_t &count)
     {
-        bool changed;
-        do {
-            changed = false;
-            for (uint32_t i = 0; i < n; ++i) {
+        while (leaf_n) {
+            for (uint32_t i = next(leaf_set, leaf_n, 0); i < n; i = next(leaf_set, leaf_n, i + 1)) {
+                take(leaf_set, leaf_n, i);
                 if (deg[i] != 1) continue;
-                changed = true;
                 for (uint32_t j = src[i]; j < src[i + 1]; ++j) {
                     if (deg[dst[j]] != -1) {
                         remove(dst[j]);
@@ -122,16 +182,19 @@ struct Approx {
                     }
                 }
             }
-        } while (changed);
+        }
     }
 
     // triangleReductionRule(CSRGraph&, unsigned&) (src/auxFunctions.cpp:71-127)
+    // (each application restarts at the lowest candidate, as the reference's
+    // scan restarts at vertex 0)
     void triangle(uint32_t &count)
     {
         bool changed;
         do {
             changed = false;
-            for (uint32_t i = 0; i < n; ++i) {
+            for (uint32_t i = next(tri_set, tri_n, 0); i < n; i = next(tri_set, tri_n, i + 1)) {
+                take(tri_set, tri_n, i);
                 if (deg[i] != 2) continue;
                 uint32_t a = 0, b = 0;
                 bool have_a = false, keep = false;

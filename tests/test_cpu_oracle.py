@@ -436,6 +436,63 @@ def test_product_upper_bounds_match_reference(vec):
     assert O.cover_is_valid(og, cove) and int(cove.sum()) == ue
 
 
+def _bound_cases():
+    """Graph families whose rule passes exercise the candidate sets of the
+    product's approximations: leaves in cascades (paths, trees, stars),
+    degree-2 chains and triangles (cycles, triangle chains, 3-regular),
+    dense and sparse random graphs, disjoint unions, one BA n = 20,000."""
+    rng = np.random.default_rng(4242)
+    for n in (1, 2, 3, 5, 13, 60):
+        yield n, [(i, i + 1) for i in range(n - 1)]
+        if n >= 3:
+            yield n, [(i, (i + 1) % n) for i in range(n)]
+        yield n, [(0, i) for i in range(1, n)]
+    yield 12, [(i, j) for i in range(12) for j in range(i + 1, 12)]
+    for k in (1, 9):
+        yield 2 * k + 1, [e for i in range(k) for e in ((2 * i, 2 * i + 1), (2 * i + 1, 2 * i + 2), (2 * i, 2 * i + 2))]
+    for _ in range(15):
+        n = int(rng.integers(2, 300))
+        yield n, [(int(rng.integers(0, i)), i) for i in range(1, n)]
+    for _ in range(40):
+        n = int(rng.integers(5, 400))
+        yield n, graphgen.erdos_renyi(n, float(rng.uniform(0.005, 0.2)), int(rng.integers(1 << 30)))
+    for _ in range(20):
+        n, m = int(rng.integers(8, 800)), int(rng.integers(1, 6))
+        yield n, graphgen.barabasi_albert(n, m, int(rng.integers(1 << 30)))
+    for _ in range(8):
+        yield 120, graphgen.regular3_rewired(120, int(rng.integers(1 << 30)))
+    for _ in range(8):
+        off, edges = 0, []
+        for _ in range(int(rng.integers(2, 10))):
+            n = int(rng.integers(2, 30))
+            e = np.asarray(graphgen.erdos_renyi(n, float(rng.uniform(0.1, 0.6)), int(rng.integers(1 << 30))))
+            edges += [(off + int(a), off + int(b)) for a, b in e.reshape(-1, 2)]
+            off += n
+        yield off, edges
+    yield 20000, graphgen.barabasi_albert(20000, 4, 5000)
+
+
+def test_product_upper_bounds_match_oracle_on_many_graphs():
+    """The product's candidate-driven approximations delete the same
+    vertices as the reference's full-scan passes: same U as the oracle's
+    restatement (which scans like the reference) for RemoveMax and for
+    RemoveEdge at several seeds, on every family above, with valid covers."""
+    vcgpu = _vc()
+    count = 0
+    for n, e in _bound_cases():
+        e = np.asarray(e, dtype=np.uint32).reshape(-1, 2)
+        g, og = vcgpu.csr_from_edges(n, e), O.from_edges(n, e)
+        u, cov = vcgpu.approx_remove_max(g)
+        assert u == O.remove_max(og), (n, len(e))
+        assert O.cover_is_valid(og, cov) and int(cov.sum()) == u
+        for seed in (0, 7, 1000):
+            ue, cove = vcgpu.approx_remove_edge(g, seed)
+            assert ue == O.remove_edge(og, seed), (n, len(e), seed)
+            assert O.cover_is_valid(og, cove) and int(cove.sum()) == ue
+        count += 1
+    assert count == 111
+
+
 def test_generator_checksums_are_stable():
     want = {v["spec"]["name"]: v["edge_sha256"] for v in VECS if v["kind"] == "single"}
     for name, cs in want.items():
