@@ -279,7 +279,10 @@ void vc_csr_free(vc_csr_owned *g);
 int vc_check_graph(const vc_csr *g);
 
 /* RemoveMaxApproximateMVC (src/auxFunctions.cpp:211-256).  Returns U; if
- * cover != NULL it receives the greedy cover (vertexNum bytes). */
+ * cover != NULL it receives the greedy cover (vertexNum bytes).  Both
+ * approximations give the reference's U bit for bit; their leaf and
+ * triangle passes visit only vertices whose test can have changed, in the
+ * reference's order, instead of rescanning every vertex. */
 uint32_t vc_approx_remove_max(const vc_csr *g, uint8_t *cover);
 
 /* RemoveEdgeApproximateMVC (src/auxFunctions.cpp:265-306); the reference
