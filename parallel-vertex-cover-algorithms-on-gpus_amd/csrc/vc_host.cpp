@@ -79,6 +79,11 @@ struct Approx {
     std::vector<int32_t> deg;
     std::vector<uint64_t> leaf_set, tri_set;  // candidate bitsets
     uint32_t leaf_n = 0, tri_n = 0;           // their members
+    // RemoveMax only: a max tree over the degrees (leaves at size + v) so
+    // the highest-degree vertex, lowest index first, is found in O(log n)
+    // instead of by a scan of every vertex per deletion
+    uint32_t size = 0;
+    std::vector<int32_t> maxt;
 
     explicit Approx(const vc_csr &g) : n(g.vertexNum)
     {
@@ -119,10 +124,36 @@ struct Approx {
         return std::min<uint32_t>(n, (w << 6) + (uint32_t)__builtin_ctzll(bits));
     }
 
+    void track_max()
+    {
+        size = 1;
+        while (size < n) size <<= 1;
+        maxt.assign(2ull * size, -1);
+        for (uint32_t v = 0; v < n; ++v) maxt[size + v] = deg[v];
+        for (uint32_t i = size - 1; i >= 1; --i) maxt[i] = std::max(maxt[2 * i], maxt[2 * i + 1]);
+    }
+    void set_max(uint32_t v)
+    {
+        uint32_t i = size + v;
+        maxt[i] = deg[v];
+        for (i >>= 1; i >= 1; i >>= 1) maxt[i] = std::max(maxt[2 * i], maxt[2 * i + 1]);
+    }
+    // the first vertex of maximum degree (the reference's scan with a strict
+    // >), and that degree; degree 0 when no vertex has an edge left
+    uint32_t max_vertex(int32_t &d) const
+    {
+        d = maxt[1];
+        if (d <= 0) return 0;
+        uint32_t i = 1;
+        while (i < size) i = maxt[2 * i] == d ? 2 * i : 2 * i + 1;
+        return i - size;
+    }
+
     // x's degree changed: x and the degree-2 vertices of its range may now
     // pass a rule's test
     void touched(uint32_t x)
     {
+        if (size) set_max(x);
         if (deg[x] == 1) add(leaf_set, leaf_n, x);
         if (deg[x] == 2) add(tri_set, tri_n, x);
         for (uint32_t j = src[x]; j < src[x + 1]; ++j)
@@ -332,19 +363,14 @@ uint32_t vc_approx_remove_max(const vc_csr *g, uint8_t *cover)
 {
     if (!g) return 0;
     Approx a(*g);
+    a.track_max();
     uint32_t count = 0;
     for (;;) {
         a.leaf(count);
         a.triangle(count);
-        uint32_t best_v = 0;
         int32_t best_d = 0;
-        for (uint32_t v = 0; v < a.n; ++v) {
-            if (a.deg[v] > best_d) {
-                best_d = a.deg[v];
-                best_v = v;
-            }
-        }
-        if (best_d == 0) break;
+        const uint32_t best_v = a.max_vertex(best_d);
+        if (best_d <= 0) break;
         a.remove(best_v);
         ++count;
     }
